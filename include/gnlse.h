@@ -1,0 +1,178 @@
+/*
+ * gnlse.h -- C ABI of the MI355X (gfx950) GNLSE propagation engine.
+ *
+ * The engine replaces pyofss's fibre stepping path (KarinaSaraeva/
+ * pyofssChanged): Fibre.__call__ -> Stepper -> Solver.rk4ip/ss_symmetric ->
+ * Linearity.exp_lin + Nonlinearity.non -> field.fft/ifft. pyofss is pure
+ * Python and has no FFI of its own; the boundary is cut at
+ * Stepper.__call__ (pyofss/modules/stepper.py:147-156) so that a whole
+ * propagation runs on the device and only traces come back. Each entry below
+ * names the reference function whose behaviour it reproduces.
+ *
+ * Conventions
+ *   - complex arrays are interleaved (re, im) float64, C-contiguous, exactly
+ *     numpy complex128; spectral arrays are in FFT order.
+ *   - every call returns GNLSE_OK (0) or a negative error code and sets a
+ *     thread-local message readable with gnlse_last_error().
+ *   - "host" entry points take host pointers and copy through device buffers
+ *     owned by the context; "_device" entry points take device pointers.
+ *   - one context per thread at a time; calls on one context must be
+ *     serialised by the caller.
+ */
+#ifndef GNLSE_H
+#define GNLSE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gnlse_ctx gnlse_ctx;
+
+enum {
+  GNLSE_OK = 0,
+  GNLSE_ERR_ARG = -1,
+  GNLSE_ERR_HIP = -2,
+  GNLSE_ERR_UNSUPPORTED = -3,
+  GNLSE_ERR_STATE = -4
+};
+
+/* integrators, pyofss/modules/solver.py:339-354 and :266-275 */
+enum { GNLSE_METHOD_RK4IP = 0, GNLSE_METHOD_SS_SYMMETRIC = 1 };
+
+/* nonlinear operator variants, pyofss/modules/nonlinearity.py:131-152 */
+enum {
+  GNLSE_NL_KERR = 0,  /* default_f / default_exp_f                 :266-270 */
+  GNLSE_NL_ALL = 1,   /* default_f_all / default_exp_f_all         :201-221 */
+  GNLSE_NL_SS = 2,    /* default_f_with_ss / default_exp_f_with_ss :223-234 */
+  GNLSE_NL_RS = 3,    /* default_f_with_rs / default_exp_f_with_rs :236-248 */
+  GNLSE_NL_SS_RS = 4  /* default_f_with_ss_and_rs / exp variant    :250-264 */
+};
+
+/* per-instance termination status; map to pyofss/modules/stepper.py:33-46 */
+enum {
+  GNLSE_STATUS_OK = 0,
+  GNLSE_STATUS_SMALL_STEP = 1,      /* SmallStepSizeError          stepper.py:350-351 */
+  GNLSE_STATUS_SUITABLE_STEP = 2,   /* SuitableStepSizeError       stepper.py:353-354 */
+  GNLSE_STATUS_MAX_STEPS = 3,       /* MaximumStepsAllocatedError  stepper.py:376 */
+  GNLSE_STATUS_TRACE_OVERFLOW = 4,  /* trace buffer too small (engine-only) */
+  GNLSE_STATUS_REPLAY_EXHAUSTED = 5 /* replay list shorter than the run (engine-only) */
+};
+
+/* Adaptive step-doubling controller constants, stepper.py:133-139, 256-274 */
+typedef struct gnlse_adaptive_params {
+  double length;          /* fibre length (km) */
+  double h0;              /* initial step, stepper.py:256-264 */
+  double local_error;     /* target relative local error */
+  double safety;          /* 0.9 */
+  double max_factor;      /* 10.0 */
+  double min_factor;      /* 0.2 */
+  double step_size_min;   /* 1e-37 */
+  int64_t traces;         /* Stepper.traces (>= 1) */
+  int64_t total_attempts; /* 1000 */
+  int64_t steps_max;      /* 100000 */
+  int32_t eta;            /* local error order of the method (5 rk4ip, 3 ss_symmetric) */
+  int32_t reserved;
+} gnlse_adaptive_params;
+
+/* Per-instance results of an adaptive run. */
+typedef struct gnlse_instance_result {
+  double z;               /* final z */
+  double h;               /* next step size proposed by the controller */
+  int64_t n_attempts;     /* attempted steps (3 integrator calls each) */
+  int64_t n_accepted;     /* accepted steps */
+  int64_t n_traces;       /* traces written (including the initial one) */
+  int32_t status;         /* GNLSE_STATUS_* */
+  int32_t reserved;
+} gnlse_instance_result;
+
+const char* gnlse_last_error(void);
+int gnlse_version(void);
+/* 1 if grid size n has a compiled kernel plan */
+int gnlse_grid_supported(int64_t n);
+int gnlse_device_count(int* count);
+
+int gnlse_ctx_create(int device, gnlse_ctx** out);
+int gnlse_ctx_destroy(gnlse_ctx* ctx);
+
+/* Grid size (samples). Replaces Domain.total_samples, pyofss/domain.py:202 */
+int gnlse_set_grid(gnlse_ctx* ctx, int64_t n);
+
+/* Linear factor i*sum_n beta_n Domega^n/n! - alpha/2 (complex128[n], FFT
+ * order). Replaces Linearity.default_linearity, linearity.py:162-186 */
+int gnlse_set_linear(gnlse_ctx* ctx, const double* factor);
+
+/* Nonlinear operator. Replaces Nonlinearity.__call__/default_nonlinearity,
+ * nonlinearity.py:160-192. omega: float64[n] (omega - omega_c, FFT order);
+ * nl_mult: complex128[n] = i*gamma*(1 + omega*ss) (NL_ALL only); h_R:
+ * complex128[n] = fft(h_R)*window_t (NL_ALL only). NULL where unused. */
+int gnlse_set_nonlinear(gnlse_ctx* ctx, int kind, double gamma, double f_R, double ss, double rs,
+                        const double* omega, const double* nl_mult, const double* h_R);
+
+/* Saturable gain coupled into every linear sub-step. Replaces
+ * Amplifier.factor (amplifier.py:149-158) as called from linearity.py:248-272.
+ * gain_log = ln(10^(0.1*G_dB)); profile: float64[n] FFT order; simpson_w:
+ * float64[n] weights with energy = sum_i w_i |A_i|^2 (field.py:161-172). */
+int gnlse_set_gain(gnlse_ctx* ctx, int enabled, double gain_log, double length, double E_sat,
+                   const double* profile, const double* simpson_w);
+
+/* sign=+1: pyofss field.fft (scipy ifft, 1/N); sign=-1: field.ifft
+ * (scipy fft). Replaces field.py:109-136. */
+int gnlse_fft(gnlse_ctx* ctx, int sign, int64_t batch, const double* in, double* out);
+
+/* Linearity.exp_lin(A, h) = ifft(exp(factor*h)[*exp(gain)] * fft(A)),
+ * linearity.py:241-256 */
+int gnlse_apply_linear(gnlse_ctx* ctx, double h, int64_t batch, const double* in, double* out);
+
+/* Linearity.lin(A, z) = ifft(factor * fft(A)), linearity.py:238-239 */
+int gnlse_apply_linear_factor(gnlse_ctx* ctx, int64_t batch, const double* in, double* out);
+
+/* Nonlinearity.non(A, z), nonlinearity.py:201-207, 223-228, 236-241, 250-256, 266-267 */
+int gnlse_apply_nonlinear(gnlse_ctx* ctx, int64_t batch, const double* in, double* out);
+
+/* Nonlinearity.exp_non(A, h, B), nonlinearity.py:209-221, 230-234, 243-248, 258-264, 269-270 */
+int gnlse_apply_exp_nonlinear(gnlse_ctx* ctx, double h, int64_t batch, const double* A, const double* B,
+                              double* out);
+
+/* One integrator call Solver.<method>(A, z, h) with exp(factor*h/2)
+ * recomputed (solver.py:266-275, 339-354 through linearity.py:241-256). */
+int gnlse_step(gnlse_ctx* ctx, int method, double h, int64_t batch, const double* in, double* out);
+
+/* Fixed-step propagation, Stepper.standard_stepper (stepper.py:164-222) with
+ * the cached linear factor of linearity.py:220-224/258-272. Traces: after
+ * step i when trace_every > 0 and i % trace_every == 0, as stepper.py:206-209;
+ * traces is complex128[batch][max_traces][n]. n_traces: int64[batch]. */
+int gnlse_propagate_fixed(gnlse_ctx* ctx, int method, int64_t batch, double h, int64_t nsteps,
+                          int64_t trace_every, int64_t max_traces, const double* A_in, double* A_out,
+                          double* traces, int64_t* n_traces);
+
+/* Adaptive propagation, Stepper.adaptive_stepper (stepper.py:247-376) with
+ * per-instance state. Optional replay (replay_h != NULL): attempt k uses step
+ * replay_h[k] and acceptance replay_accept[k] instead of the controller's.
+ * Traces (max_traces > 0): trace_A complex128[batch][max_traces][n], trace_z,
+ * step_z, step_h float64[batch][max_traces] (step_* valid where a trace was
+ * stored after a step). Attempt log (log_cap > 0): log_h, log_delta
+ * float64[batch][log_cap], log_accept uint8[batch][log_cap]. */
+int gnlse_propagate_adaptive(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
+                             const double* replay_h, const uint8_t* replay_accept, int64_t replay_len,
+                             const double* A_in, double* A_out, int64_t max_traces, double* trace_A,
+                             double* trace_z, double* step_z, double* step_h, int64_t log_cap, double* log_h,
+                             double* log_delta, uint8_t* log_accept, gnlse_instance_result* results);
+
+/* Device-memory helpers and device-resident adaptive propagation (inputs
+ * already in HBM; no traces, no log). results is a host array. */
+int gnlse_malloc(gnlse_ctx* ctx, int64_t bytes, void** dptr);
+int gnlse_free(gnlse_ctx* ctx, void* dptr);
+int gnlse_memcpy_htod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int gnlse_memcpy_dtoh(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
+                                    const double* d_A_in, double* d_A_out, gnlse_instance_result* results);
+
+/* Duration in ms of the last propagation kernel, from HIP events recorded on
+ * the context's stream around the launch. */
+int gnlse_last_kernel_ms(gnlse_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GNLSE_H */

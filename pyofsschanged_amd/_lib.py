@@ -1,0 +1,338 @@
+"""ctypes binding of libgnlse_hip.so (the C ABI of include/gnlse.h).
+
+The shared library is built in-tree by ``python -m pyofsschanged_amd.build``.
+Every numerical entry point runs on the GPU; if the library or a HIP device is
+missing, the calls raise ``EngineUnavailableError`` instead of computing
+anything on the host.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgnlse_hip.so")
+
+METHOD_RK4IP = 0
+METHOD_SS_SYMMETRIC = 1
+
+NL_KERR = 0
+NL_ALL = 1
+NL_SS = 2
+NL_RS = 3
+NL_SS_RS = 4
+
+STATUS_OK = 0
+STATUS_SMALL_STEP = 1
+STATUS_SUITABLE_STEP = 2
+STATUS_MAX_STEPS = 3
+STATUS_TRACE_OVERFLOW = 4
+STATUS_REPLAY_EXHAUSTED = 5
+
+
+class EngineUnavailableError(RuntimeError):
+    """The HIP engine cannot run: library not built or no GPU visible."""
+
+
+class GNLSEError(RuntimeError):
+    """A call into the HIP engine failed."""
+
+
+class AdaptiveParams(ctypes.Structure):
+    _fields_ = [
+        ("length", ctypes.c_double),
+        ("h0", ctypes.c_double),
+        ("local_error", ctypes.c_double),
+        ("safety", ctypes.c_double),
+        ("max_factor", ctypes.c_double),
+        ("min_factor", ctypes.c_double),
+        ("step_size_min", ctypes.c_double),
+        ("traces", ctypes.c_int64),
+        ("total_attempts", ctypes.c_int64),
+        ("steps_max", ctypes.c_int64),
+        ("eta", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class InstanceResult(ctypes.Structure):
+    _fields_ = [
+        ("z", ctypes.c_double),
+        ("h", ctypes.c_double),
+        ("n_attempts", ctypes.c_int64),
+        ("n_accepted", ctypes.c_int64),
+        ("n_traces", ctypes.c_int64),
+        ("status", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+
+# name -> argtypes; every function returns int
+SIGNATURES = {
+    "gnlse_version": [],
+    "gnlse_grid_supported": [_I64],
+    "gnlse_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "gnlse_ctx_create": [_INT, ctypes.POINTER(_P)],
+    "gnlse_ctx_destroy": [_P],
+    "gnlse_set_grid": [_P, _I64],
+    "gnlse_set_linear": [_P, _P],
+    "gnlse_set_nonlinear": [_P, _INT, _D, _D, _D, _D, _P, _P, _P],
+    "gnlse_set_gain": [_P, _INT, _D, _D, _D, _P, _P],
+    "gnlse_fft": [_P, _INT, _I64, _P, _P],
+    "gnlse_apply_linear": [_P, _D, _I64, _P, _P],
+    "gnlse_apply_linear_factor": [_P, _I64, _P, _P],
+    "gnlse_apply_nonlinear": [_P, _I64, _P, _P],
+    "gnlse_apply_exp_nonlinear": [_P, _D, _I64, _P, _P, _P],
+    "gnlse_step": [_P, _INT, _D, _I64, _P, _P],
+    "gnlse_propagate_fixed": [_P, _INT, _I64, _D, _I64, _I64, _I64, _P, _P, _P, _P],
+    "gnlse_propagate_adaptive": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _I64, _P, _P, _I64, _P,
+                                 _P, _P, _P, _I64, _P, _P, _P, _P],
+    "gnlse_malloc": [_P, _I64, ctypes.POINTER(_P)],
+    "gnlse_free": [_P, _P],
+    "gnlse_memcpy_htod": [_P, _P, _P, _I64],
+    "gnlse_memcpy_dtoh": [_P, _P, _P, _I64],
+    "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
+    "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path=LIB_PATH):
+    """Load the engine library (no GPU needed to load it)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise EngineUnavailableError(
+                f"{path} is not built; run `python -m pyofsschanged_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_int
+        lib.gnlse_last_error.argtypes = []
+        lib.gnlse_last_error.restype = ctypes.c_char_p
+        _lib = lib
+        return lib
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        msg = lib.gnlse_last_error().decode(errors="replace")
+        raise GNLSEError(f"{what} failed ({rc}): {msg}")
+
+
+def device_count():
+    lib = load_library()
+    n = ctypes.c_int(0)
+    rc = lib.gnlse_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def grid_supported(n):
+    return bool(load_library().gnlse_grid_supported(int(n)))
+
+
+def _c128(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    if n is not None and a.shape[-1] != n:
+        raise ValueError(f"field has {a.shape[-1]} samples, the context grid has {n}")
+    return a
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.shape[-1] != n:
+        raise ValueError(f"array has {a.shape[-1]} samples, the context grid has {n}")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One engine context: a grid, operator arrays and device buffers on one
+    GPU. Calls are serialised by a lock (the C context is not thread-safe)."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        if device_count() < 1:
+            raise EngineUnavailableError("no HIP device is visible to the GNLSE engine")
+        h = ctypes.c_void_p()
+        _check(self.lib, self.lib.gnlse_ctx_create(int(device), ctypes.byref(h)), "gnlse_ctx_create")
+        self._h = h
+        self.device = device
+        self.n = None
+        self.lock = threading.RLock()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gnlse_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        with self.lock:
+            _check(self.lib, getattr(self.lib, name)(self._h, *args), name)
+
+    # -- set-up -------------------------------------------------------------
+    def set_grid(self, n):
+        self._call("gnlse_set_grid", int(n))
+        self.n = int(n)
+
+    def set_linear(self, factor):
+        f = _c128(np.broadcast_to(np.asarray(factor, dtype=np.complex128), (self.n,)), self.n)
+        self._call("gnlse_set_linear", _ptr(f))
+
+    def set_nonlinear(self, kind, gamma, f_R=0.0, ss=0.0, rs=0.0, omega=None, nl_mult=None, h_R=None):
+        om = None if omega is None else _f64(omega, self.n)
+        nm = None if nl_mult is None else _c128(nl_mult, self.n)
+        hr = None if h_R is None else _c128(h_R, self.n)
+        self._call("gnlse_set_nonlinear", int(kind), float(gamma), float(f_R), float(ss), float(rs), _ptr(om),
+                   _ptr(nm), _ptr(hr))
+
+    def set_gain(self, enabled, gain_log=0.0, length=1.0, E_sat=1.0, profile=None, simpson_w=None):
+        pr = None if profile is None else _f64(profile, self.n)
+        sw = None if simpson_w is None else _f64(simpson_w, self.n)
+        self._call("gnlse_set_gain", int(bool(enabled)), float(gain_log), float(length), float(E_sat), _ptr(pr),
+                   _ptr(sw))
+
+    # -- operators ------------------------------------------------------------
+    def _batched(self, A):
+        A = _c128(A, self.n)
+        shape = A.shape
+        return A.reshape(-1, self.n), shape
+
+    def fft(self, A, sign):
+        a, shape = self._batched(A)
+        out = np.empty_like(a)
+        self._call("gnlse_fft", int(sign), a.shape[0], _ptr(a), _ptr(out))
+        return out.reshape(shape)
+
+    def apply_linear(self, A, h):
+        a, shape = self._batched(A)
+        out = np.empty_like(a)
+        self._call("gnlse_apply_linear", float(h), a.shape[0], _ptr(a), _ptr(out))
+        return out.reshape(shape)
+
+    def apply_linear_factor(self, A):
+        a, shape = self._batched(A)
+        out = np.empty_like(a)
+        self._call("gnlse_apply_linear_factor", a.shape[0], _ptr(a), _ptr(out))
+        return out.reshape(shape)
+
+    def apply_nonlinear(self, A):
+        a, shape = self._batched(A)
+        out = np.empty_like(a)
+        self._call("gnlse_apply_nonlinear", a.shape[0], _ptr(a), _ptr(out))
+        return out.reshape(shape)
+
+    def apply_exp_nonlinear(self, A, h, B):
+        a, shape = self._batched(A)
+        b, _ = self._batched(B)
+        if b.shape != a.shape:
+            raise ValueError("A and B must have the same shape")
+        out = np.empty_like(a)
+        self._call("gnlse_apply_exp_nonlinear", float(h), a.shape[0], _ptr(a), _ptr(b), _ptr(out))
+        return out.reshape(shape)
+
+    def step(self, method, A, h):
+        a, shape = self._batched(A)
+        out = np.empty_like(a)
+        self._call("gnlse_step", int(method), float(h), a.shape[0], _ptr(a), _ptr(out))
+        return out.reshape(shape)
+
+    # -- propagation ------------------------------------------------------------
+    def propagate_fixed(self, method, A, h, nsteps, trace_every=0, max_traces=0):
+        a, shape = self._batched(A)
+        B = a.shape[0]
+        out = np.empty_like(a)
+        traces = np.empty((B, max(max_traces, 0), self.n), np.complex128)
+        ntr = np.zeros(B, np.int64)
+        self._call("gnlse_propagate_fixed", int(method), B, float(h), int(nsteps), int(trace_every),
+                   int(max_traces), _ptr(a), _ptr(out), _ptr(traces) if max_traces > 0 else None,
+                   _ptr(ntr) if max_traces > 0 else None)
+        return out.reshape(shape), traces, ntr
+
+    def propagate_adaptive(self, method, A, params, max_traces=0, log_cap=0, replay=None):
+        a, shape = self._batched(A)
+        B = a.shape[0]
+        out = np.empty_like(a)
+        tA = np.empty((B, max_traces, self.n), np.complex128)
+        tz = np.zeros((B, max_traces))
+        sz = np.zeros((B, max_traces))
+        sh = np.zeros((B, max_traces))
+        lh = np.zeros((B, log_cap))
+        ld = np.zeros((B, log_cap))
+        la = np.zeros((B, log_cap), np.uint8)
+        res = (InstanceResult * B)()
+        rh = ra = None
+        rlen = 0
+        if replay is not None:
+            rh = _f64(replay[0])
+            ra = np.ascontiguousarray(replay[1], dtype=np.uint8)
+            rlen = len(rh)
+        mt = max_traces > 0
+        lg = log_cap > 0
+        self._call("gnlse_propagate_adaptive", int(method), B, ctypes.byref(params), _ptr(rh), _ptr(ra), int(rlen),
+                   _ptr(a), _ptr(out), int(max_traces), _ptr(tA) if mt else None, _ptr(tz) if mt else None,
+                   _ptr(sz) if mt else None, _ptr(sh) if mt else None, int(log_cap), _ptr(lh) if lg else None,
+                   _ptr(ld) if lg else None, _ptr(la) if lg else None, res)
+        results = [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
+                    "n_traces": r.n_traces, "status": r.status} for r in res]
+        return {"A": out.reshape(shape), "traces": tA, "trace_z": tz, "step_z": sz, "step_h": sh,
+                "log_h": lh, "log_delta": ld, "log_accept": la, "results": results}
+
+    # -- device-resident -----------------------------------------------------------
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self._call("gnlse_malloc", int(nbytes), ctypes.byref(p))
+        return p
+
+    def free(self, p):
+        self._call("gnlse_free", p)
+
+    def htod(self, dptr, host):
+        host = np.ascontiguousarray(host)
+        self._call("gnlse_memcpy_htod", dptr, _ptr(host), host.nbytes)
+
+    def dtoh(self, host, dptr):
+        self._call("gnlse_memcpy_dtoh", _ptr(host), dptr, host.nbytes)
+
+    def propagate_adaptive_device(self, method, batch, params, d_in, d_out):
+        res = (InstanceResult * int(batch))()
+        self._call("gnlse_propagate_adaptive_device", int(method), int(batch), ctypes.byref(params), d_in, d_out, res)
+        return [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
+                 "n_traces": r.n_traces, "status": r.status} for r in res]
+
+    def last_kernel_ms(self):
+        v = ctypes.c_double()
+        self._call("gnlse_last_kernel_ms", ctypes.byref(v))
+        return v.value
+
+
+_default = {}
+
+
+def default_context(device=0):
+    """A process-wide context for standalone operator calls (field.fft, ...)."""
+    ctx = _default.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _default[device] = ctx
+    return ctx

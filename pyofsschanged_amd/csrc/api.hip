@@ -1,0 +1,547 @@
+// C ABI of the GNLSE engine (include/gnlse.h): contexts, operator upload,
+// device buffers, and dispatch to the per-grid launch tables.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace gn {
+const LaunchTable* table_256();
+const LaunchTable* table_512();
+const LaunchTable* table_1024();
+const LaunchTable* table_2048();
+const LaunchTable* table_4096();
+const LaunchTable* table_8192();
+
+const LaunchTable* find_table(long long n) {
+  const LaunchTable* all[] = {table_256(), table_512(), table_1024(), table_2048(), table_4096(), table_8192()};
+  for (const LaunchTable* t : all)
+    if (t->n == n) return t;
+  return nullptr;
+}
+}  // namespace gn
+
+using namespace gn;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(GNLSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RET_IF(x)                  \
+  do {                             \
+    int r_ = (x);                  \
+    if (r_ != GNLSE_OK) return r_; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct gnlse_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int64_t n = 0;
+  const LaunchTable* tab = nullptr;
+  DevBuf lin, efix, nlmul, hr, omega, gprof, simpw, tw;
+  double efix_h = NAN;
+  bool have_lin = false, have_nl = false;
+  Ops ops{};
+  DevBuf ws, io_in, io_out, io_b, tr_A, tr_z, st_z, st_h, lg_h, lg_d, lg_a, rp_h, rp_a, res, ntr;
+};
+
+static int ensure(DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return GNLSE_OK;
+  if (b.p) HIPCHK(hipFree(b.p));
+  b.p = nullptr;
+  b.bytes = 0;
+  HIPCHK(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  return GNLSE_OK;
+}
+static void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+static int upload(gnlse_ctx* c, DevBuf& b, const void* src, size_t bytes) {
+  RET_IF(ensure(b, bytes));
+  HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+// exp(+2 pi i m / N) in extended precision, rounded to double.
+static int build_twiddles(gnlse_ctx* c) {
+  const int64_t n = c->n;
+  std::vector<double> tw(2 * n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int64_t m = 0; m < n; ++m) {
+    const long double a = two_pi * (long double)m / (long double)n;
+    tw[2 * m] = (double)cosl(a);
+    tw[2 * m + 1] = (double)sinl(a);
+  }
+  return upload(c, c->tw, tw.data(), tw.size() * sizeof(double));
+}
+
+static void refresh_ops(gnlse_ctx* c) {
+  c->ops.lin = (const double2*)c->lin.p;
+  c->ops.efix = (const double2*)c->efix.p;
+  c->ops.nlmul = (const double2*)c->nlmul.p;
+  c->ops.hr = (const double2*)c->hr.p;
+  c->ops.omega = (const double*)c->omega.p;
+  c->ops.tw = (const double2*)c->tw.p;
+  c->ops.gprof = (const double*)c->gprof.p;
+  c->ops.simpw = (const double*)c->simpw.p;
+}
+
+static int check_ready(gnlse_ctx* c, bool need_lin, bool need_nl) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  if (c->n == 0 || !c->tab) return fail(GNLSE_ERR_STATE, "gnlse_set_grid has not been called");
+  if (need_lin && !c->have_lin) return fail(GNLSE_ERR_STATE, "gnlse_set_linear has not been called");
+  if (need_nl && !c->have_nl) return fail(GNLSE_ERR_STATE, "gnlse_set_nonlinear has not been called");
+  return GNLSE_OK;
+}
+
+static int ensure_ws(gnlse_ctx* c, int64_t batch) {
+  return ensure(c->ws, (size_t)batch * NSLOT * c->n * sizeof(double2));
+}
+
+static int timed_begin(gnlse_ctx* c) {
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  return GNLSE_OK;
+}
+static int timed_end(gnlse_ctx* c, hipError_t launch) {
+  if (launch != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(launch));
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return GNLSE_OK;
+}
+
+static int io_prepare(gnlse_ctx* c, int64_t batch, const double* in, DevBuf& dst) {
+  if (batch <= 0) return fail(GNLSE_ERR_ARG, "batch must be positive");
+  if (batch > 0x7fffffff) return fail(GNLSE_ERR_ARG, "batch too large");
+  if (!in) return fail(GNLSE_ERR_ARG, "null input");
+  const size_t bytes = (size_t)batch * c->n * sizeof(double2);
+  RET_IF(ensure(dst, bytes));
+  HIPCHK(hipMemcpyAsync(dst.p, in, bytes, hipMemcpyHostToDevice, c->stream));
+  return GNLSE_OK;
+}
+static int io_finish(gnlse_ctx* c, int64_t batch, double* out) {
+  if (!out) return fail(GNLSE_ERR_ARG, "null output");
+  const size_t bytes = (size_t)batch * c->n * sizeof(double2);
+  HIPCHK(hipMemcpyAsync(out, c->io_out.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+static bool method_ok(int m) { return m == GNLSE_METHOD_RK4IP || m == GNLSE_METHOD_SS_SYMMETRIC; }
+
+extern "C" {
+
+const char* gnlse_last_error(void) { return g_err.c_str(); }
+int gnlse_version(void) { return 1; }
+int gnlse_grid_supported(int64_t n) { return find_table(n) ? 1 : 0; }
+
+int gnlse_device_count(int* count) {
+  if (!count) return fail(GNLSE_ERR_ARG, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(GNLSE_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  *count = n;
+  return GNLSE_OK;
+}
+
+int gnlse_ctx_create(int device, gnlse_ctx** out) {
+  if (!out) return fail(GNLSE_ERR_ARG, "null out");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(GNLSE_ERR_HIP, std::string("no HIP device available: ") + hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(GNLSE_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  gnlse_ctx* c = new gnlse_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(GNLSE_ERR_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return GNLSE_OK;
+}
+
+int gnlse_ctx_destroy(gnlse_ctx* c) {
+  if (!c) return GNLSE_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->lin,  &c->efix,  &c->nlmul, &c->hr,   &c->omega, &c->gprof, &c->simpw, &c->tw,
+                    &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
+                    &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr};
+  for (DevBuf* b : bufs) release(*b);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GNLSE_OK;
+}
+
+int gnlse_set_grid(gnlse_ctx* c, int64_t n) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  const LaunchTable* t = find_table(n);
+  if (!t) return fail(GNLSE_ERR_UNSUPPORTED, "grid size " + std::to_string(n) + " has no kernel plan");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->n != n) {
+    c->n = n;
+    c->tab = t;
+    c->have_lin = c->have_nl = false;
+    c->efix_h = NAN;
+    RET_IF(build_twiddles(c));
+    std::vector<double> zeros(2 * n, 0.0);  // every operator pointer stays valid
+    RET_IF(upload(c, c->nlmul, zeros.data(), 2 * n * sizeof(double)));
+    RET_IF(upload(c, c->hr, zeros.data(), 2 * n * sizeof(double)));
+    RET_IF(upload(c, c->omega, zeros.data(), n * sizeof(double)));
+    RET_IF(upload(c, c->gprof, zeros.data(), n * sizeof(double)));
+    RET_IF(upload(c, c->simpw, zeros.data(), n * sizeof(double)));
+    RET_IF(ensure(c->efix, 2 * n * sizeof(double)));
+    c->ops.gain = 0;
+    refresh_ops(c);
+  }
+  return GNLSE_OK;
+}
+
+int gnlse_set_linear(gnlse_ctx* c, const double* factor) {
+  RET_IF(check_ready(c, false, false));
+  if (!factor) return fail(GNLSE_ERR_ARG, "null factor");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(upload(c, c->lin, factor, 2 * c->n * sizeof(double)));
+  c->have_lin = true;
+  c->efix_h = NAN;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_set_nonlinear(gnlse_ctx* c, int kind, double gamma, double f_R, double ss, double rs, const double* omega,
+                        const double* nl_mult, const double* h_R) {
+  RET_IF(check_ready(c, false, false));
+  if (kind < GNLSE_NL_KERR || kind > GNLSE_NL_SS_RS) return fail(GNLSE_ERR_ARG, "unknown nonlinearity kind");
+  if (kind != GNLSE_NL_KERR && !omega) return fail(GNLSE_ERR_ARG, "omega is required for this nonlinearity");
+  if (kind == GNLSE_NL_ALL && (!nl_mult || !h_R))
+    return fail(GNLSE_ERR_ARG, "nl_mult and h_R are required for NL_ALL");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n = (size_t)c->n;
+  if (omega) RET_IF(upload(c, c->omega, omega, n * sizeof(double)));
+  if (nl_mult) RET_IF(upload(c, c->nlmul, nl_mult, 2 * n * sizeof(double)));
+  if (h_R) RET_IF(upload(c, c->hr, h_R, 2 * n * sizeof(double)));
+  c->ops.nl_kind = kind;
+  c->ops.fac = make_double2(0.0, gamma);
+  c->ops.f_R = f_R;
+  c->ops.ss = ss;
+  c->ops.rs = rs;
+  c->have_nl = true;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, double E_sat, const double* profile,
+                   const double* simpson_w) {
+  RET_IF(check_ready(c, false, false));
+  HIPCHK(hipSetDevice(c->device));
+  if (!enabled) {
+    c->ops.gain = 0;
+    return GNLSE_OK;
+  }
+  if (!profile || !simpson_w) return fail(GNLSE_ERR_ARG, "profile and simpson_w are required");
+  if (!(E_sat > 0.0) || !(length > 0.0)) return fail(GNLSE_ERR_ARG, "E_sat and length must be positive");
+  const size_t n = (size_t)c->n;
+  RET_IF(upload(c, c->gprof, profile, n * sizeof(double)));
+  RET_IF(upload(c, c->simpw, simpson_w, n * sizeof(double)));
+  c->ops.gain = 1;
+  c->ops.gain_log = gain_log;
+  c->ops.gain_len2 = 2.0 * length;
+  c->ops.esat = E_sat;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_fft(gnlse_ctx* c, int sign, int64_t batch, const double* in, double* out) {
+  RET_IF(check_ready(c, false, false));
+  if (sign != 1 && sign != -1) return fail(GNLSE_ERR_ARG, "sign must be +1 or -1");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->fft(c->stream, c->ops, sign, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_apply_linear(gnlse_ctx* c, double h, int64_t batch, const double* in, double* out) {
+  RET_IF(check_ready(c, true, false));
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->linear(c->stream, c->ops, h, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p,
+                                     (double2*)c->ws.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_apply_linear_factor(gnlse_ctx* c, int64_t batch, const double* in, double* out) {
+  RET_IF(check_ready(c, true, false));
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->linfac(c->stream, c->ops, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_apply_nonlinear(gnlse_ctx* c, int64_t batch, const double* in, double* out) {
+  RET_IF(check_ready(c, false, true));
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->nonlin(c->stream, c->ops, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p,
+                                     (double2*)c->ws.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_apply_exp_nonlinear(gnlse_ctx* c, double h, int64_t batch, const double* A, const double* B, double* out) {
+  RET_IF(check_ready(c, false, true));
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, A, c->io_in));
+  RET_IF(io_prepare(c, batch, B, c->io_b));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->exp_nonlin(c->stream, c->ops, h, batch, (const double2*)c->io_in.p,
+                                         (const double2*)c->io_b.p, (double2*)c->io_out.p, (double2*)c->ws.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_step(gnlse_ctx* c, int method, double h, int64_t batch, const double* in, double* out) {
+  RET_IF(check_ready(c, true, true));
+  if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->step(c->stream, c->ops, method, h, batch, (const double2*)c->io_in.p,
+                                   (double2*)c->io_out.p, (double2*)c->ws.p)));
+  return io_finish(c, batch, out);
+}
+
+int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, int64_t nsteps, int64_t trace_every,
+                          int64_t max_traces, const double* A_in, double* A_out, double* traces,
+                          int64_t* n_traces) {
+  RET_IF(check_ready(c, true, true));
+  if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
+  if (nsteps < 0 || trace_every < 0 || max_traces < 0) return fail(GNLSE_ERR_ARG, "negative count");
+  if (max_traces > 0 && (!traces || !n_traces)) return fail(GNLSE_ERR_ARG, "null trace outputs");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t n = c->n;
+  // linearity.py:220-224, 258-262: the cached factor is exp(factor*h/2)
+  const double hh = 0.5 * h;
+  if (!(c->efix_h == hh)) {
+    hipError_t e = c->tab->make_exp(c->stream, c->ops, hh, (double2*)c->efix.p);
+    if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("make_exp: ") + hipGetErrorString(e));
+    c->efix_h = hh;
+  }
+  RET_IF(io_prepare(c, batch, A_in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * n * sizeof(double2)));
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(ensure(c->tr_A, (size_t)batch * max_traces * n * sizeof(double2)));
+  RET_IF(ensure(c->ntr, (size_t)batch * sizeof(int64_t)));
+  FixedParams prm;
+  prm.h = h;
+  prm.nsteps = nsteps;
+  prm.trace_every = trace_every;
+  prm.max_traces = max_traces;
+  prm.tr_A = (double2*)c->tr_A.p;
+  prm.n_traces = (long long*)c->ntr.p;
+  RET_IF(timed_begin(c));
+  RET_IF(timed_end(c, c->tab->fixed(c->stream, c->ops, method, prm, batch, (const double2*)c->io_in.p,
+                                    (double2*)c->io_out.p, (double2*)c->ws.p)));
+  if (max_traces > 0) {
+    HIPCHK(hipMemcpyAsync(traces, c->tr_A.p, (size_t)batch * max_traces * n * sizeof(double2), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(n_traces, c->ntr.p, (size_t)batch * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+  }
+  return io_finish(c, batch, A_out);
+}
+
+static int adaptive_params(int method, const gnlse_adaptive_params* p, AdaptiveParams& prm) {
+  if (!p) return fail(GNLSE_ERR_ARG, "null params");
+  if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
+  if (!(p->length > 0.0) || !(p->h0 > 0.0) || p->traces < 1 || p->eta < 2 || p->total_attempts < 1)
+    return fail(GNLSE_ERR_ARG, "invalid adaptive parameters");
+  std::memset(&prm, 0, sizeof(prm));
+  prm.length = p->length;
+  prm.h0 = p->h0;
+  prm.tol = p->local_error;
+  prm.safety = p->safety;
+  prm.max_factor = p->max_factor;
+  prm.min_factor = p->min_factor;
+  prm.hmin = p->step_size_min;
+  prm.inv_eta = 1.0 / (double)p->eta;
+  const double f_eta = std::pow(2.0, (double)p->eta - 1.0);  // stepper.py:272-274
+  prm.f_alpha = f_eta / (f_eta - 1.0);
+  prm.f_beta = 1.0 / (f_eta - 1.0);
+  prm.step_storage = p->length / (double)p->traces;
+  prm.traces = p->traces;
+  prm.total_attempts = p->total_attempts;
+  prm.steps_max = p->steps_max;
+  return GNLSE_OK;
+}
+
+static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm, const double2* in,
+                           double2* out) {
+  RET_IF(ensure_ws(c, batch));
+  RET_IF(timed_begin(c));
+  return timed_end(c, c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p));
+}
+
+int gnlse_propagate_adaptive(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
+                             const double* replay_h, const uint8_t* replay_accept, int64_t replay_len,
+                             const double* A_in, double* A_out, int64_t max_traces, double* trace_A,
+                             double* trace_z, double* step_z, double* step_h, int64_t log_cap, double* log_h,
+                             double* log_delta, uint8_t* log_accept, gnlse_instance_result* results) {
+  RET_IF(check_ready(c, true, true));
+  AdaptiveParams prm;
+  RET_IF(adaptive_params(method, p, prm));
+  if (!results) return fail(GNLSE_ERR_ARG, "null results");
+  if (max_traces < 0 || log_cap < 0 || replay_len < 0) return fail(GNLSE_ERR_ARG, "negative count");
+  if (max_traces > 0 && (!trace_A || !trace_z || !step_z || !step_h))
+    return fail(GNLSE_ERR_ARG, "null trace outputs");
+  if (log_cap > 0 && (!log_h || !log_delta || !log_accept)) return fail(GNLSE_ERR_ARG, "null log outputs");
+  if (replay_h && (!replay_accept || replay_len <= 0)) return fail(GNLSE_ERR_ARG, "invalid replay");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t n = c->n;
+  RET_IF(io_prepare(c, batch, A_in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * n * sizeof(double2)));
+  RET_IF(ensure(c->tr_A, (size_t)batch * max_traces * n * sizeof(double2)));
+  RET_IF(ensure(c->tr_z, (size_t)batch * max_traces * sizeof(double)));
+  RET_IF(ensure(c->st_z, (size_t)batch * max_traces * sizeof(double)));
+  RET_IF(ensure(c->st_h, (size_t)batch * max_traces * sizeof(double)));
+  RET_IF(ensure(c->lg_h, (size_t)batch * log_cap * sizeof(double)));
+  RET_IF(ensure(c->lg_d, (size_t)batch * log_cap * sizeof(double)));
+  RET_IF(ensure(c->lg_a, (size_t)batch * log_cap));
+  RET_IF(ensure(c->res, (size_t)batch * sizeof(gnlse_instance_result)));
+  prm.max_traces = max_traces;
+  prm.log_cap = log_cap;
+  prm.tr_A = (double2*)c->tr_A.p;
+  prm.tr_z = (double*)c->tr_z.p;
+  prm.st_z = (double*)c->st_z.p;
+  prm.st_h = (double*)c->st_h.p;
+  prm.log_h = (double*)c->lg_h.p;
+  prm.log_delta = (double*)c->lg_d.p;
+  prm.log_acc = (unsigned char*)c->lg_a.p;
+  prm.res = (gnlse_instance_result*)c->res.p;
+  if (replay_h) {
+    RET_IF(upload(c, c->rp_h, replay_h, (size_t)replay_len * sizeof(double)));
+    RET_IF(upload(c, c->rp_a, replay_accept, (size_t)replay_len));
+    prm.replay_h = (const double*)c->rp_h.p;
+    prm.replay_accept = (const unsigned char*)c->rp_a.p;
+    prm.replay_len = replay_len;
+  }
+  RET_IF(launch_adaptive(c, method, batch, prm, (const double2*)c->io_in.p, (double2*)c->io_out.p));
+  auto d2h = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return GNLSE_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return GNLSE_OK;
+  };
+  if (max_traces > 0) {
+    RET_IF(d2h(trace_A, c->tr_A.p, (size_t)batch * max_traces * n * sizeof(double2)));
+    RET_IF(d2h(trace_z, c->tr_z.p, (size_t)batch * max_traces * sizeof(double)));
+    RET_IF(d2h(step_z, c->st_z.p, (size_t)batch * max_traces * sizeof(double)));
+    RET_IF(d2h(step_h, c->st_h.p, (size_t)batch * max_traces * sizeof(double)));
+  }
+  if (log_cap > 0) {
+    RET_IF(d2h(log_h, c->lg_h.p, (size_t)batch * log_cap * sizeof(double)));
+    RET_IF(d2h(log_delta, c->lg_d.p, (size_t)batch * log_cap * sizeof(double)));
+    RET_IF(d2h(log_accept, c->lg_a.p, (size_t)batch * log_cap));
+  }
+  RET_IF(d2h(results, c->res.p, (size_t)batch * sizeof(gnlse_instance_result)));
+  return io_finish(c, batch, A_out);
+}
+
+int gnlse_malloc(gnlse_ctx* c, int64_t bytes, void** dptr) {
+  if (!c || !dptr || bytes <= 0) return fail(GNLSE_ERR_ARG, "invalid malloc request");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMalloc(dptr, (size_t)bytes));
+  return GNLSE_OK;
+}
+int gnlse_free(gnlse_ctx* c, void* dptr) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (dptr) HIPCHK(hipFree(dptr));
+  return GNLSE_OK;
+}
+int gnlse_memcpy_htod(gnlse_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return fail(GNLSE_ERR_ARG, "invalid copy");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+int gnlse_memcpy_dtoh(gnlse_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return fail(GNLSE_ERR_ARG, "invalid copy");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
+                                    const double* d_A_in, double* d_A_out, gnlse_instance_result* results) {
+  RET_IF(check_ready(c, true, true));
+  AdaptiveParams prm;
+  RET_IF(adaptive_params(method, p, prm));
+  if (batch <= 0 || batch > 0x7fffffff || !d_A_in || !d_A_out || !results)
+    return fail(GNLSE_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(ensure(c->res, (size_t)batch * sizeof(gnlse_instance_result)));
+  prm.res = (gnlse_instance_result*)c->res.p;
+  RET_IF(launch_adaptive(c, method, batch, prm, (const double2*)d_A_in, (double2*)d_A_out));
+  HIPCHK(hipMemcpyAsync(results, c->res.p, (size_t)batch * sizeof(gnlse_instance_result), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_last_kernel_ms(gnlse_ctx* c, double* ms) {
+  if (!c || !ms) return fail(GNLSE_ERR_ARG, "null argument");
+  *ms = c->last_ms;
+  return GNLSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,386 @@
+// Kernels of the GNLSE engine and their per-grid launch tables.
+//
+// Execution model: one workgroup owns one propagation ("instance"). The
+// propagation kernels are persistent: the whole fixed-step or adaptive run of
+// an instance, including the step-doubling controller, its reductions, the
+// accept/reject decision and trace capture, happens inside one launch. A
+// batch of instances is one launch with one workgroup per instance.
+//
+// Each grid size is compiled in its own translation unit (grid_<N>.hip),
+// which instantiates Launcher<N, T> and exports its table.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/gnlse.h"
+#include "ops.hpp"
+
+namespace gn {
+
+struct FixedParams {
+  double h;
+  long long nsteps, trace_every, max_traces;
+  double2* tr_A;
+  long long* n_traces;
+};
+
+struct AdaptiveParams {
+  double length, h0, tol, safety, max_factor, min_factor, hmin, inv_eta, f_alpha, f_beta, step_storage;
+  long long traces, total_attempts, steps_max, max_traces, log_cap, replay_len;
+  const double* replay_h;
+  const unsigned char* replay_accept;
+  double2* tr_A;
+  double* tr_z;
+  double* st_z;
+  double* st_h;
+  double* log_h;
+  double* log_delta;
+  unsigned char* log_acc;
+  gnlse_instance_result* res;
+};
+
+struct LaunchTable {
+  int n, t;
+  hipError_t (*fft)(hipStream_t, const Ops&, int sign, long long batch, const double2* in, double2* out);
+  hipError_t (*make_exp)(hipStream_t, const Ops&, double h, double2* out);
+  hipError_t (*linear)(hipStream_t, const Ops&, double h, long long batch, const double2* in, double2* out,
+                       double2* ws);
+  hipError_t (*linfac)(hipStream_t, const Ops&, long long batch, const double2* in, double2* out);
+  hipError_t (*nonlin)(hipStream_t, const Ops&, long long batch, const double2* in, double2* out, double2* ws);
+  hipError_t (*exp_nonlin)(hipStream_t, const Ops&, double h, long long batch, const double2* A, const double2* B,
+                           double2* out, double2* ws);
+  hipError_t (*step)(hipStream_t, const Ops&, int method, double h, long long batch, const double2* in,
+                     double2* out, double2* ws);
+  hipError_t (*fixed)(hipStream_t, const Ops&, int method, const FixedParams&, long long batch, const double2* in,
+                      double2* out, double2* ws);
+  hipError_t (*adaptive)(hipStream_t, const Ops&, int method, const AdaptiveParams&, long long batch,
+                         const double2* in, double2* out, double2* ws);
+};
+
+const LaunchTable* find_table(long long n);
+
+#define COMMA ,
+#define GN_SHARED(N, T, K)                       \
+  __shared__ double lds[FFT<N, T>::LDS_DOUBLES]; \
+  __shared__ double red[(T) / 64 + 2];           \
+  Dev<N, T, K> d(lds, red, ops)
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) k_fft(Ops ops, int sign, const double2* __restrict__ in, double2* __restrict__ out) {
+  GN_SHARED(N, T, NL_KERR);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2 x[P];
+  d.load(x, in + b * N);
+  if (sign > 0) d.fft(x);
+  else d.ifft(x);
+  d.store(out + b * N, x);
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T) k_make_exp(Ops ops, double h, double2* __restrict__ out) {
+  GN_SHARED(N, T, NL_KERR);
+  d.make_exp(out, h);
+}
+
+template <int N, int T>
+__global__ void __launch_bounds__(T)
+    k_linear(Ops ops, double h, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
+  GN_SHARED(N, T, NL_KERR);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  d.make_exp(ws + S_EH * N, h);
+  double2 x[P];
+  d.load(x, in + b * N);
+  d.linear(x, ws + S_EH * N, h);
+  d.store(out + b * N, x);
+}
+
+// linearity.py:238-239 default_f: ifft(factor * fft(A))
+template <int N, int T>
+__global__ void __launch_bounds__(T) k_linfac(Ops ops, const double2* __restrict__ in, double2* __restrict__ out) {
+  GN_SHARED(N, T, NL_KERR);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2 x[P];
+  d.load(x, in + b * N);
+  d.spectral(x, MUL_ARRAY, ops.lin, 0.0);
+  d.store(out + b * N, x);
+}
+
+template <int N, int T, int K>
+__global__ void __launch_bounds__(T)
+    k_nonlin(Ops ops, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
+  GN_SHARED(N, T, K);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  double2 x[P];
+  d.load(x, in + b * N);
+  d.nonlin(x, ws + S_U * N, ws + S_T * N);
+  d.store(out + b * N, x);
+}
+
+template <int N, int T, int K>
+__global__ void __launch_bounds__(T) k_exp_nonlin(Ops ops, double h, const double2* __restrict__ A,
+                                                  const double2* __restrict__ B, double2* __restrict__ out,
+                                                  double2* ws_all) {
+  GN_SHARED(N, T, K);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  double2 x[P];
+  d.load(x, A + b * N);
+  d.store(ws + S_A * N, x);
+  d.load(x, B + b * N);
+  d.exp_nonlin(x, ws + S_A * N, h, ws + S_U * N, ws + S_T * N);
+  d.store(out + b * N, x);
+}
+
+template <int N, int T, int METHOD, int K>
+__global__ void __launch_bounds__(T)
+    k_step(Ops ops, double h, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
+  GN_SHARED(N, T, K);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  double2 x[P];
+  d.make_exp(ws + S_EH * N, 0.5 * h);
+  d.load(x, in + b * N);
+  d.store(ws + S_A * N, x);
+  integrator<METHOD>(d, x, ws + S_A * N, ws + S_EH * N, h, ws);
+  d.store(out + b * N, x);
+}
+
+// stepper.py:164-222 with the cached exp(factor*h/2) in ops.efix
+template <int N, int T, int METHOD, int K>
+__global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const double2* __restrict__ in,
+                                             double2* __restrict__ out, double2* ws_all) {
+  GN_SHARED(N, T, K);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  double2* sA = ws + S_A * N;
+  double2 x[P];
+  d.load(x, in + b * N);
+  d.store(sA, x);
+  long long n_tr = 0;
+  for (long long i = 0; i < prm.nsteps; ++i) {
+    integrator<METHOD>(d, x, sA, ops.efix, prm.h, ws);
+    d.store(sA, x);
+    if (prm.trace_every > 0 && (i % prm.trace_every) == 0) {  // stepper.py:206-209
+      if (n_tr < prm.max_traces) d.store(prm.tr_A + (b * prm.max_traces + n_tr) * N, x);
+      ++n_tr;
+    }
+  }
+  d.store(out + b * N, x);
+  if (threadIdx.x == 0 && prm.n_traces) prm.n_traces[b] = n_tr;
+}
+
+// stepper.py:247-376: step doubling, local extrapolation and the controller,
+// all per instance on the device.
+template <int N, int T, int METHOD, int K>
+__global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
+                                                double2* __restrict__ out, double2* ws_all) {
+  GN_SHARED(N, T, K);
+  constexpr int P = N / T;
+  const size_t b = blockIdx.x;
+  double2* ws = ws_all + b * NSLOT * N;
+  double2* sA = ws + S_A * N;
+  double2* sC = ws + S_C * N;
+  double2* sH = ws + S_H * N;
+  double2* sEQ = ws + S_EQ * N;
+  double2* sEH = ws + S_EH * N;
+  double2 x[P];
+  d.load(x, in + b * N);
+  d.store(sA, x);
+
+  double z = 0.0, h = prm.h0, z_ignore = prm.step_storage;
+  long long n_tr = 0, n_att = 0, n_acc = 0;
+  int status = GNLSE_STATUS_OK;
+  bool finished = false;
+
+  auto store_trace = [&](double zt, double ht) {
+    if (n_tr < prm.max_traces) {
+      d.store(prm.tr_A + (b * prm.max_traces + n_tr) * N, x);
+      if (threadIdx.x == 0) {
+        prm.tr_z[b * prm.max_traces + n_tr] = zt;
+        prm.st_z[b * prm.max_traces + n_tr] = zt;
+        prm.st_h[b * prm.max_traces + n_tr] = ht;
+      }
+      ++n_tr;
+      return true;
+    }
+    return false;
+  };
+
+  if (prm.traces != 1 && prm.max_traces > 0) store_trace(0.0, 0.0);  // stepper.py:284-285
+
+  for (long long s = 1; s < prm.steps_max; ++s) {
+    if ((z + h) > prm.length) h = prm.length - z;  // stepper.py:291-292
+    bool accepted = false;
+    for (long long ta = 0; ta < prm.total_attempts; ++ta) {
+      if (prm.replay_h) {
+        if (n_att >= prm.replay_len) {
+          status = GNLSE_STATUS_REPLAY_EXHAUSTED;
+          break;
+        }
+        h = prm.replay_h[n_att];
+      }
+      const double h_half = 0.5 * h;
+      d.make_exp(sEH, 0.5 * h);
+      d.make_exp(sEQ, 0.5 * h_half);
+      // stepper.py:305-308: coarse (h), then half and fine (h/2 twice)
+#pragma unroll 1
+      for (int c = 0; c < 3; ++c) {
+        const double2* src = (c == 2) ? sH : sA;
+        const double2* E = (c == 0) ? sEH : sEQ;
+        integrator<METHOD>(d, x, src, E, c == 0 ? h : h_half, ws);
+        if (c == 0) d.store(sC, x);
+        else if (c == 1) d.store(sH, x);
+      }
+      // stepper.py:230-245 relative_local_error
+      double sd = 0.0, sf = 0.0, bad = 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 f = x[k];
+        const double2 df = csub(f, sC[d.idx(k)]);
+        sd += abs2(df);
+        sf += abs2(f);
+        if (!(isfinite(f.x) && isfinite(f.y))) bad = 1.0;
+      }
+      bad = d.block_sum(bad);
+      sd = d.block_sum(sd);
+      sf = d.block_sum(sf);
+      double delta;
+      if (bad > 0.0) delta = 1.0;
+      else if (sf != 0.0) delta = sqrt(sd) / sqrt(sf);
+      else delta = sqrt(sd);
+      // stepper.py:313-325 controller
+      const double h_temp = h;
+      if (delta > 0.0) {
+        const double ratio = prm.tol / delta;
+        const double factor = prm.safety * pow(ratio, prm.inv_eta);
+        h = h_temp * fmin(prm.max_factor, fmax(prm.min_factor, factor));
+      } else {
+        h = h_temp * prm.max_factor;
+      }
+      const bool acc = prm.replay_h ? (prm.replay_accept[n_att] != 0) : (delta < 2.0 * prm.tol);
+      if (threadIdx.x == 0 && n_att < prm.log_cap) {
+        prm.log_h[b * prm.log_cap + n_att] = h_temp;
+        prm.log_delta[b * prm.log_cap + n_att] = delta;
+        prm.log_acc[b * prm.log_cap + n_att] = acc ? 1 : 0;
+      }
+      ++n_att;
+      if (acc) {
+        z += h_temp;
+        ++n_acc;
+        // local extrapolation, stepper.py:336-338
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+          x[k] = csub(cscale(x[k], prm.f_alpha), cscale(sC[d.idx(k)], prm.f_beta));
+        d.store(sA, x);
+        if (z > z_ignore) {  // stepper.py:342-345
+          if (prm.max_traces > 0 && !store_trace(z, h_temp)) status = GNLSE_STATUS_TRACE_OVERFLOW;
+          z_ignore += prm.step_storage;
+        }
+        accepted = true;
+        break;
+      } else if (h < prm.hmin) {  // stepper.py:349-351
+        status = GNLSE_STATUS_SMALL_STEP;
+        break;
+      }
+    }
+    if (status != GNLSE_STATUS_OK) break;
+    if (!accepted) {  // stepper.py:353-354
+      status = GNLSE_STATUS_SUITABLE_STEP;
+      break;
+    }
+    if (z >= prm.length) {  // stepper.py:357
+      finished = true;
+      break;
+    }
+  }
+  if (!finished && status == GNLSE_STATUS_OK) status = GNLSE_STATUS_MAX_STEPS;
+  d.load(x, sA);
+  d.store(out + b * N, x);
+  if (threadIdx.x == 0) {
+    gnlse_instance_result r;
+    r.z = z;
+    r.h = h;
+    r.n_attempts = n_att;
+    r.n_accepted = n_acc;
+    r.n_traces = n_tr;
+    r.status = status;
+    r.reserved = 0;
+    prm.res[b] = r;
+  }
+}
+
+// Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
+// operator get their own instantiations; the other variants share one with
+// runtime selection.
+#define GN_BY_KIND(ops, KERNEL, ...)                                                          \
+  do {                                                                                       \
+    if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR>), __VA_ARGS__);       \
+    else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL>), __VA_ARGS__);    \
+    else hipLaunchKernelGGL((KERNEL, NL_GENERIC>), __VA_ARGS__);                             \
+  } while (0)
+
+template <int N, int T>
+struct Launcher {
+  static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out) {
+    hipLaunchKernelGGL((k_fft<N, T>), dim3(batch), dim3(T), 0, s, o, sign, in, out);
+    return hipGetLastError();
+  }
+  static hipError_t make_exp(hipStream_t s, const Ops& o, double h, double2* out) {
+    hipLaunchKernelGGL((k_make_exp<N, T>), dim3(1), dim3(T), 0, s, o, h, out);
+    return hipGetLastError();
+  }
+  static hipError_t linear(hipStream_t s, const Ops& o, double h, long long batch, const double2* in, double2* out,
+                           double2* ws) {
+    hipLaunchKernelGGL((k_linear<N, T>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    return hipGetLastError();
+  }
+  static hipError_t linfac(hipStream_t s, const Ops& o, long long batch, const double2* in, double2* out) {
+    hipLaunchKernelGGL((k_linfac<N, T>), dim3(batch), dim3(T), 0, s, o, in, out);
+    return hipGetLastError();
+  }
+  static hipError_t nonlin(hipStream_t s, const Ops& o, long long batch, const double2* in, double2* out,
+                           double2* ws) {
+    GN_BY_KIND(o, k_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, in, out, ws);
+    return hipGetLastError();
+  }
+  static hipError_t exp_nonlin(hipStream_t s, const Ops& o, double h, long long batch, const double2* A,
+                               const double2* B, double2* out, double2* ws) {
+    GN_BY_KIND(o, k_exp_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, h, A, B, out, ws);
+    return hipGetLastError();
+  }
+  static hipError_t step(hipStream_t s, const Ops& o, int method, double h, long long batch, const double2* in,
+                         double2* out, double2* ws) {
+    if (method == M_RK4IP) GN_BY_KIND(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    else GN_BY_KIND(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    return hipGetLastError();
+  }
+  static hipError_t fixed(hipStream_t s, const Ops& o, int method, const FixedParams& p, long long batch,
+                          const double2* in, double2* out, double2* ws) {
+    if (method == M_RK4IP) GN_BY_KIND(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    return hipGetLastError();
+  }
+  static hipError_t adaptive(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
+                             const double2* in, double2* out, double2* ws) {
+    if (method == M_RK4IP)
+      GN_BY_KIND(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    return hipGetLastError();
+  }
+  static constexpr LaunchTable table = {N, T, &fft, &make_exp, &linear, &linfac, &nonlin, &exp_nonlin, &step, &fixed, &adaptive};
+};
+
+}  // namespace gn
+
+#define GN_DEFINE_GRID(NN, TT) \
+  namespace gn {               \
+  const LaunchTable* table_##NN() { return &Launcher<NN, TT>::table; } \
+  }

@@ -1,0 +1,421 @@
+// Device-side GNLSE operators for one field per workgroup.
+//
+// Each thread owns the points tid + T*k of the field (k < P = N/T): pointwise
+// work and the per-instance global slots are touched only at owned indices,
+// so they need no barrier; the FFTs exchange data through LDS.
+//
+// The operators restate, in this register layout, the arithmetic of
+//   pyofss/modules/linearity.py:241-272    exp_lin (with amplifier coupling)
+//   pyofss/modules/nonlinearity.py:201-270 non / exp_non variants
+//   pyofss/modules/amplifier.py:149-158    saturable gain factor
+//   pyofss/modules/solver.py:266-275, 339-354  ss_symmetric, rk4ip
+// with the pyofss transform convention of pyofss/field.py:109-136:
+//   fft(A)  = (1/N) sum_n A_n exp(+2 pi i k n / N)   = F+(A)/N
+//   ifft(A) =       sum_k A_k exp(-2 pi i k n / N)   = conj(F+(conj(A)))
+// Every spectral operator ifft(m * fft(y)) is one routine with a single
+// F+ body run twice, so a kernel holds few copies of the transform.
+#pragma once
+#include "fft.hpp"
+
+namespace gn {
+
+enum NlKind { NL_GENERIC = -1, NL_KERR = 0, NL_ALL = 1, NL_SS = 2, NL_RS = 3, NL_SS_RS = 4 };
+enum Method { M_RK4IP = 0, M_SS_SYMMETRIC = 1 };
+// spectral multiplier kinds
+enum MulKind { MUL_ARRAY = 0, MUL_OMEGA = 1, MUL_IOMEGA = 2, MUL_GAIN = 3 };
+
+struct Ops {
+  int nl_kind;
+  int gain;
+  double f_R, ss, rs;
+  double2 fac;              // nonlinear factor i*gamma (nonlinearity.py:187-192)
+  const double2* lin;       // linear factor i*D(omega) - alpha/2, FFT order
+  const double2* efix;      // exp(lin * h/2) of the fixed-step path
+  const double2* nlmul;     // fac * (1 + omega*ss), FFT order
+  const double2* hr;        // fft(h_R) * window_t, FFT order
+  const double* omega;      // omega - omega_c, FFT order
+  const double2* tw;        // exp(+2 pi i m / N), m < N
+  double gain_log;          // ln(10^(0.1 G))
+  double gain_len2;         // 2 * amplifier length
+  double esat;              // saturation energy
+  const double* gprof;      // spectral gain profile, FFT order
+  const double* simpw;      // Simpson weights of the energy integral
+};
+
+__device__ __forceinline__ double2 cexp(double2 z) {
+  double s, c;
+  sincos(z.y, &s, &c);
+  const double e = exp(z.x);
+  return make_double2(e * c, e * s);
+}
+
+// numpy's complex division (Smith's method), used for ss/B.
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  const double ac = fabs(b.x), ad = fabs(b.y);
+  if (ac >= ad) {
+    if (ac == 0.0 && ad == 0.0) return make_double2(a.x / ac, a.y / ad);
+    const double r = b.y / b.x, den = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+  }
+  const double r = b.x / b.y, den = b.x * r + b.y;
+  return make_double2((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+
+__device__ __forceinline__ double abs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// Component-wise select. A ternary between two double2 array elements can be
+// lowered to a select of their addresses, which pins both arrays in scratch
+// memory; selecting the scalars keeps them in registers.
+__device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
+  return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+// KIND fixes the nonlinear operator at compile time (NL_KERR, NL_ALL) or
+// leaves it to ops.nl_kind (NL_GENERIC); a fixed kind lets the compiler drop
+// the other operators and their register live ranges.
+template <int N, int T, int KIND>
+struct Dev {
+  using F = FFT<N, T>;
+  static constexpr int P = N / T;
+  static constexpr double INVN = 1.0 / N;  // exact: N is a power of two
+  static constexpr int NW = T / 64;
+  double* lds;
+  double* red;
+  const Ops& op;
+  typename F::Twiddles tw;
+
+  __device__ Dev(double* l, double* r, const Ops& o) : lds(l), red(r), op(o) { F::load_twiddles(tw, o.tw); }
+
+  // Owned index of register k. The thread index goes through opaque() so
+  // that slot addresses are formed at each use instead of being hoisted out
+  // of the step loops and kept live for the whole kernel.
+  __device__ __forceinline__ unsigned idx(int k) const { return (unsigned)opaque((int)threadIdx.x) + T * k; }
+  __device__ __forceinline__ int kind() const { return KIND == NL_GENERIC ? op.nl_kind : KIND; }
+
+  __device__ __forceinline__ void load(double2 (&x)[P], const double2* __restrict__ g) const {
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = g[idx(k)];
+  }
+  __device__ __forceinline__ void store(double2* __restrict__ g, const double2 (&x)[P]) const {
+#pragma unroll
+    for (int k = 0; k < P; ++k) g[idx(k)] = x[k];
+  }
+
+  // pyofss fft / ifft of a register field (standalone transforms)
+  __device__ __forceinline__ void fft(double2 (&x)[P]) {
+    F::plus(x, lds, tw);
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = cscale(x[k], INVN);
+  }
+  __device__ __forceinline__ void ifft(double2 (&x)[P]) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = conjg(x[k]);
+    F::plus(x, lds, tw);
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = conjg(x[k]);
+  }
+
+  // y <- ifft(m * fft(y)); m selected by `mul` (MUL_GAIN: m = exp(g*prof)*arr).
+  __device__ __forceinline__ void spectral(double2 (&y)[P], int mul, const double2* __restrict__ arr, double g) {
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) {
+      F::plus(y, lds, tw);
+      if (it == 0) {
+        if (mul == MUL_ARRAY) {
+#pragma unroll
+          for (int k = 0; k < P; ++k) y[k] = conjg(cmul(arr[idx(k)], cscale(y[k], INVN)));
+        } else if (mul == MUL_OMEGA) {
+#pragma unroll
+          for (int k = 0; k < P; ++k) y[k] = conjg(cscale(cscale(y[k], INVN), op.omega[idx(k)]));
+        } else if (mul == MUL_IOMEGA) {
+#pragma unroll
+          for (int k = 0; k < P; ++k) y[k] = conjg(cmul(make_double2(0.0, op.omega[idx(k)]), cscale(y[k], INVN)));
+        } else {
+#pragma unroll
+          for (int k = 0; k < P; ++k)
+            y[k] = conjg(cscale(cmul(arr[idx(k)], cscale(y[k], INVN)), exp(g * op.gprof[idx(k)])));
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) y[k] = conjg(y[k]);
+  }
+
+  // Sum over the workgroup; every thread gets the same value (fixed order).
+  __device__ __forceinline__ double block_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w];
+    return s;
+  }
+
+  // linearity.py:241-272 with amplifier.py:149-158:
+  // x <- ifft(exp(G0/(1+E(x)/E_sat) * profile) * E * fft(x))
+  __device__ __forceinline__ void linear(double2 (&x)[P], const double2* __restrict__ E, double h) {
+    if (op.gain) {
+      double e = 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) e += op.simpw[idx(k)] * abs2(x[k]);
+      const double En = block_sum(e);
+      const double G0 = (op.gain_log * h) / op.gain_len2;
+      spectral(x, MUL_GAIN, E, G0 / (1.0 + En / op.esat));
+    } else {
+      spectral(x, MUL_ARRAY, E, 0.0);
+    }
+  }
+
+  // nonlinearity.py non(A): x <- N(x). The input is parked in slot sX while
+  // the transforms run, so only one field is live in registers; sT holds the
+  // self-steepening term of NL_SS_RS.
+  __device__ __forceinline__ void nonlin(double2 (&x)[P], double2* __restrict__ sX, double2* __restrict__ sT) {
+    const int kind = this->kind();
+    const double2 fac = op.fac;
+    if (kind == NL_KERR) {  // :266-267  factor * |A|^2 * A
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double p = abs2(x[k]);
+        x[k] = cmul(make_double2(fac.x * p, fac.y * p), x[k]);
+      }
+      return;
+    }
+    const int npass = (kind == NL_ALL || kind == NL_SS_RS) ? 2 : 1;
+    double2 y[P];
+    if (kind == NL_ALL || kind == NL_RS) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        y[k] = make_double2(abs2(x[k]), 0.0);
+        sX[idx(k)] = x[k];
+      }
+    } else if (kind == NL_SS) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        y[k] = cscale(x[k], abs2(x[k]));  // term_spm of :225
+        sX[idx(k)] = y[k];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        y[k] = cscale(x[k], abs2(x[k]));  // |A|^2 A of :253
+        sX[idx(k)] = x[k];
+      }
+    }
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+      int mul = MUL_ARRAY;
+      const double2* arr = op.hr;
+      if (kind == NL_ALL) arr = pass == 0 ? op.hr : op.nlmul;
+      else if (kind == NL_SS) mul = MUL_OMEGA;
+      else if (kind == NL_RS) mul = MUL_IOMEGA;
+      else mul = pass == 0 ? MUL_OMEGA : MUL_IOMEGA;
+      spectral(y, mul, arr, 0.0);
+      if (pass == 0 && npass == 2) {
+        if (kind == NL_ALL) {  // :205
+          const double a1 = 1.0 - op.f_R;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            const double2 a = sX[idx(k)];
+            y[k] = cadd(cscale(cscale(a, a1), abs2(a)), cmul(cscale(a, op.f_R), y[k]));
+          }
+        } else {  // NL_SS_RS: keep term_ss, start term_rs
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            sT[idx(k)] = cscale(y[k], op.ss);
+            y[k] = make_double2(abs2(sX[idx(k)]), 0.0);
+          }
+        }
+      }
+    }
+    if (kind == NL_ALL) {  // :207
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = y[k];
+    } else if (kind == NL_SS) {  // :228
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = cmul(fac, cadd(sX[idx(k)], cscale(y[k], op.ss)));
+    } else {  // :241 (RS), :256 (SS_RS adds term_ss outside the factor)
+      const bool with_ss = (kind == NL_SS_RS);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 a = sX[idx(k)];
+        const double2 tr = cscale(y[k], op.rs);
+        const double2 v = cmul(cmul(fac, make_double2(abs2(a) + tr.x, tr.y)), a);
+        double2 ts = make_double2(0.0, 0.0);
+        if (with_ss) ts = sT[idx(k)];
+        x[k] = cadd(v, ts);
+      }
+    }
+  }
+
+  // nonlinearity.py exp_non(A, h, B): B <- exp(h*factor*(...)) * B, with A
+  // read from the global slot gA; B is parked in sX during the transforms.
+  __device__ __forceinline__ void exp_nonlin(double2 (&B)[P], const double2* __restrict__ gA, double h, double2* __restrict__ sX,
+                             double2* __restrict__ sT) {
+    const int kind = this->kind();
+    const double2 hf = make_double2(h * op.fac.x, h * op.fac.y);
+    if (kind == NL_KERR) {  // :269-270
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double p = abs2(gA[idx(k)]);
+        B[k] = cmul(cexp(make_double2(hf.x * p, hf.y * p)), B[k]);
+      }
+      return;
+    }
+    const int npass = (kind == NL_ALL || kind == NL_SS_RS) ? 2 : 1;
+    double2 y[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      sX[idx(k)] = B[k];
+      const double2 a = gA[idx(k)];
+      const double p = abs2(a);
+      y[k] = sel(kind == NL_ALL || kind == NL_RS, make_double2(p, 0.0), cscale(a, p));
+    }
+#pragma unroll 1
+    for (int pass = 0; pass < npass; ++pass) {
+      int mul;
+      if (kind == NL_ALL) mul = pass == 0 ? MUL_ARRAY : MUL_OMEGA;
+      else if (kind == NL_SS) mul = MUL_OMEGA;
+      else if (kind == NL_RS) mul = MUL_IOMEGA;
+      else mul = pass == 0 ? MUL_OMEGA : MUL_IOMEGA;
+      spectral(y, mul, op.hr, 0.0);
+      if (pass == 0 && npass == 2) {
+        if (kind == NL_ALL) {  // :212-214 term_all, then term_all * A
+          const double a1 = 1.0 - op.f_R;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            const double2 a = gA[idx(k)];
+            const double2 ta = cadd(make_double2(a1 * abs2(a), 0.0), cscale(y[k], op.f_R));
+            sT[idx(k)] = ta;
+            y[k] = cmul(ta, a);
+          }
+        } else {  // NL_SS_RS :261 term_ss, then |A|^2 for term_rs
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            sT[idx(k)] = cmul(cdiv(make_double2(op.ss, 0.0), sX[idx(k)]), y[k]);
+            y[k] = make_double2(abs2(gA[idx(k)]), 0.0);
+          }
+        }
+      }
+    }
+    if (kind == NL_ALL) {  // :216, :221
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 b = sX[idx(k)];
+        const double2 q = sel(hypot(b.x, b.y) > 1e-15, cdiv(make_double2(op.ss, 0.0), b), make_double2(0.0, 0.0));
+        B[k] = cmul(cexp(cmul(hf, cadd(sT[idx(k)], cmul(q, y[k])))), b);
+      }
+    } else if (kind == NL_SS) {  // :233-234
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 b = sX[idx(k)];
+        const double2 tss = cmul(cdiv(make_double2(op.ss, 0.0), b), y[k]);
+        B[k] = cmul(cexp(cmul(hf, make_double2(abs2(gA[idx(k)]) + tss.x, tss.y))), b);
+      }
+    } else {  // :246-248 (RS), :263-264 (SS_RS)
+      const bool with_ss = (kind == NL_SS_RS);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 b = sX[idx(k)];
+        double2 ts = make_double2(0.0, 0.0);
+        if (with_ss) ts = sT[idx(k)];
+        const double2 tr = cscale(y[k], op.rs);
+        const double p = abs2(gA[idx(k)]);
+        const double2 s = make_double2(p + ts.x + tr.x, ts.y + tr.y);  // ts is zero for RS
+        B[k] = cmul(cexp(cmul(hf, s)), b);
+      }
+    }
+  }
+
+  // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
+  __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double2 f = op.lin[idx(k)];
+      E[idx(k)] = cexp(make_double2(f.x * h, f.y * h));
+    }
+  }
+};
+
+// Per-instance scratch slots (N complex each).
+enum Slot { S_A = 0, S_C, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, NSLOT };
+
+// solver.py:339-354 rk4ip as a 7-stage loop (one call site per operator).
+// Input field in slot `in`; E = exp(factor*h/2). Result in x.
+template <int N, int T, int K>
+__device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+                      double2* ws) {
+  constexpr int P = N / T;
+  double2* sAI = ws + S_AI * N;
+  double2* sK0 = ws + S_K0 * N;
+  double2* sK1 = ws + S_K1 * N;
+  double2* sT = ws + S_T * N;
+  double2* sU = ws + S_U * N;
+  const double hh = 0.5 * h;
+#pragma unroll 1
+  for (int st = 0; st < 7; ++st) {
+    // pre
+    if (st == 0 || st == 2) d.load(x, in);
+    else if (st == 3) {  // A_I + 0.5*k0
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = cadd(x[k], cscale(sK0[d.idx(k)], 0.5));
+    } else if (st == 4) {  // A_I + 0.5*k1
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = cadd(sAI[d.idx(k)], cscale(x[k], 0.5));
+    } else if (st == 6) d.load(x, sK0);
+    if (st == 1 || st == 2 || st == 5 || st == 6) d.linear(x, E, hh);
+    if (st == 0 || st == 3 || st == 4 || st == 5) {
+      d.nonlin(x, sU, sT);
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
+    }
+    // post
+    if (st == 1) d.store(sK0, x);        // k0 = L(h N(A))
+    else if (st == 2) d.store(sAI, x);   // A_I = L(A)
+    else if (st == 3) d.store(sK1, x);   // k1
+    else if (st == 4) {                  // k2 in x
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int i = d.idx(k);
+        const double2 ai = sAI[i];
+        const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], x[k]), 2.0));
+        sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+        x[k] = cadd(ai, x[k]);                                  // A_I + k2
+      }
+    } else if (st == 5) d.store(sK1, x);  // k3
+    else if (st == 6) {                   // k3/6 + L(acc)
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 k3 = sK1[d.idx(k)];
+        x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
+      }
+    }
+  }
+}
+
+// solver.py:266-275 ss_symmetric: L(h/2), exp_non with the phase of the
+// un-propagated input, L(h/2).
+template <int N, int T, int K>
+__device__ __forceinline__ void ss_symmetric(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+                             double2* ws) {
+  double2* sT = ws + S_T * N;
+  double2* sU = ws + S_U * N;
+  const double hh = 0.5 * h;
+  d.load(x, in);
+#pragma unroll 1
+  for (int st = 0; st < 3; ++st) {
+    if (st == 1) d.exp_nonlin(x, in, h, sU, sT);
+    else d.linear(x, E, hh);
+  }
+}
+
+template <int METHOD, int N, int T, int K>
+__device__ __forceinline__ void integrator(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E,
+                                           double h, double2* ws) {
+  if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws);
+  else ss_symmetric(d, x, in, E, h, ws);
+}
+
+}  // namespace gn

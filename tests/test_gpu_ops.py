@@ -1,0 +1,136 @@
+"""GPU parity of the single operators against the oracle (complex128).
+
+Tolerances (relative L2 unless stated):
+  * FFT vs the oracle's pocketfft: 1e-14; known answers: 1e-13 absolute.
+  * exp_lin, non and one integrator call: 1e-12.
+  * lin = ifft(D * fft(A)): 1e-10. D reaches ~1e8 /km at the spectral edges,
+    where the spectrum of A is at the FFT roundoff floor (~1e-16 of its
+    peak); both implementations amplify that floor by |D|, so the result is
+    only conditioned to ~1e-16 * max|D| / ||D A~||.
+  * exp_non with self-steepening: 1e-10. nonlinearity.py:230-264 divide by B,
+    which reaches the underflow region in the pulse tails; the quotient is
+    conditioned by 1/|B| there in both implementations.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import engine_context, nl_kind, oracle_ops, rel_l2, soliton_c1, supercontinuum_c2
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [256, 512, 1024, 2048, 4096, 8192]
+OP_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pyofsschanged_amd import _lib
+    if _lib.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return _lib
+
+
+@pytest.mark.parametrize("N", GRIDS)
+def test_fft_matches_oracle(lib, N):
+    rng = np.random.default_rng(N)
+    A = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    ctx = lib.Context(0)
+    ctx.set_grid(N)
+    f = ctx.fft(A, +1)
+    b = ctx.fft(A, -1)
+    print(f"N={N} fft err {rel_l2(f, O.fft(A)):.2e} ifft err {rel_l2(b, O.ifft(A)):.2e}", flush=True)
+    assert rel_l2(f, O.fft(A)) < 1e-14
+    assert rel_l2(b, O.ifft(A)) < 1e-14
+    # round trip
+    assert rel_l2(ctx.fft(f, -1), A) < 1e-14
+
+
+def test_fft_known_answers(lib):
+    N = 256
+    ctx = lib.Context(0)
+    ctx.set_grid(N)
+    n = np.arange(N)
+    # pyofss fft = (1/N) sum_n x_n exp(+2 pi i k n / N): exp(-2 pi i m n / N)
+    # maps to a unit impulse at k = m
+    for m in (0, 1, 5, 77, 128):
+        x = np.exp(-2j * np.pi * m * n / N)
+        X = ctx.fft(x, +1)
+        expect = np.zeros(N, complex)
+        expect[m] = 1.0
+        assert np.max(np.abs(X - expect)) < 1e-13
+    delta = np.zeros(N, complex)
+    delta[3] = 1.0
+    assert np.max(np.abs(ctx.fft(delta, -1) - np.exp(-2j * np.pi * 3 * n / N))) < 1e-13
+
+
+@pytest.mark.parametrize("N", [512, 4096, 8192])
+def test_linear_operator(lib, N):
+    dom, A, fib = supercontinuum_c2(N)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], alpha=0.05, gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    for h in (1e-6, 3.3e-7):
+        got = ctx.apply_linear(A, h)
+        ref = lin.default_exp_f(A, h)
+        err = rel_l2(got, ref)
+        print(f"N={N} h={h} linear err {err:.2e}", flush=True)
+        assert err < OP_TOL
+    got = ctx.apply_linear_factor(A)
+    err = rel_l2(got, lin.default_f(A, 0.0))
+    print(f"N={N} lin err {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
+KINDS = [dict(), dict(use_all=True), dict(use_all=True, self_steepening=True), dict(self_steepening=True),
+         dict(raman_scattering=True), dict(self_steepening=True, raman_scattering=True)]
+
+
+@pytest.mark.parametrize("flags", KINDS, ids=lambda d: "+".join(sorted(d)) or "kerr")
+@pytest.mark.parametrize("N", [1024, 8192])
+def test_nonlinear_operator(lib, flags, N):
+    dom, A, fib = supercontinuum_c2(N)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], **flags)
+    ctx = engine_context(dom, lin, nl, nl_kind(**flags))
+    got = ctx.apply_nonlinear(A)
+    ref = nl.non(A, 0.0)
+    err = rel_l2(got, ref)
+    print(f"N={N} {flags} non err {err:.2e}", flush=True)
+    assert err < OP_TOL
+    B = lin.default_exp_f(A, 2e-7)
+    got = ctx.apply_exp_nonlinear(A, 2e-7, B)
+    with np.errstate(all="ignore"):
+        ref = nl.exp_non(A, 2e-7, B)
+    err = rel_l2(got, ref)
+    print(f"N={N} {flags} exp_non err {err:.2e}", flush=True)
+    assert err < (1e-10 if flags.get("self_steepening") else OP_TOL)
+
+
+@pytest.mark.parametrize("method", ["rk4ip", "ss_symmetric"])
+@pytest.mark.parametrize("flags", [dict(), dict(use_all=True, self_steepening=True)], ids=["kerr", "all+ss"])
+def test_one_integrator_call(lib, method, flags):
+    dom, A, fib = supercontinuum_c2(4096)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], **flags)
+    ctx = engine_context(dom, lin, nl, nl_kind(**flags))
+
+    class F:
+        pass
+
+    f = F()
+    f.linear = lin.default_exp_f
+    f.n = nl.non
+    f.nonlinear = nl.exp_non
+    h = 5e-7
+    ref = getattr(O.Solver, method)(A, 0.0, h, f)
+    got = ctx.step(0 if method == "rk4ip" else 1, A, h)
+    err = rel_l2(got, ref)
+    print(f"{method} {flags} step err {err:.2e}", flush=True)
+    assert err < OP_TOL
+
+
+def test_soliton_operator_grid_4096(lib):
+    dom, A, fib = soliton_c1()
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    assert rel_l2(ctx.apply_nonlinear(A), nl.non(A, 0)) < OP_TOL
+    assert rel_l2(ctx.apply_linear(A, 1e-3), lin.default_exp_f(A, 1e-3)) < OP_TOL

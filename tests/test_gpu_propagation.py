@@ -1,0 +1,155 @@
+"""GPU parity of whole propagations against the oracle (complex128).
+
+Tolerance: relative L2 field error <= 1e-9 against the oracle's NumPy
+complex128 path on identical inputs, for every returned field. Adaptive runs
+must also reproduce the oracle's attempt sequence: identical accept/reject
+decisions and attempt count, step sizes to 1e-8 relative. The controller's
+error estimate is the norm of A_fine - A_coarse, a difference of two fields
+that agree to ~1e-6, so it carries ~1e-10 relative roundoff in any FP64
+implementation; h depends on it through its fifth root.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import C_NM_PS, engine_context, nl_kind, oracle_ops, rel_l2, soliton_c1, supercontinuum_c2
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pyofsschanged_amd import _lib
+    if _lib.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return _lib
+
+
+def oracle_fixed(dom, A, fib, method, steps, traces):
+    f = O.Fibre(length=fib["length"], beta=fib["beta"], gamma=fib["gamma"], method=method, total_steps=steps,
+                traces=traces)
+    out = f(dom, A.copy())
+    st = f.stepper.storage
+    return out, st.z, st.As, st.fft_total
+
+
+@pytest.mark.parametrize("method,mid", [("rk4ip", 0), ("ss_symmetric", 1)])
+def test_c1_soliton_fixed(lib, method, mid):
+    """C1: N=2 soliton over one soliton period, 1000 fixed steps, 50 traces."""
+    dom, A, fib = soliton_c1(4096)
+    ref, ref_z, ref_As, _ = oracle_fixed(dom, A, fib, method, 1000, 50)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    h = fib["length"] / 1000
+    got, tr, ntr = ctx.propagate_fixed(mid, A, h, 1000, trace_every=20, max_traces=50)
+    err = rel_l2(got, ref)
+    print(f"C1 {method}: rel L2 {err:.3e}, traces {ntr[0]}, kernel {ctx.last_kernel_ms():.2f} ms", flush=True)
+    assert err <= FIELD_TOL
+    assert ntr[0] == len(ref_As) == 50
+    for k in range(50):
+        assert rel_l2(tr[0, k], ref_As[k]) <= FIELD_TOL
+
+
+def test_c1_batch_is_bitwise_consistent(lib):
+    """Every instance of a batch computes exactly what a single run computes."""
+    dom, A, fib = soliton_c1(4096)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    h = fib["length"] / 200
+    rng = np.random.default_rng(0)
+    batch = np.stack([A * (1 + 0.01 * k) + 1e-3 * rng.standard_normal(A.size) for k in range(5)])
+    got, _, _ = ctx.propagate_fixed(0, batch, h, 200)
+    for k in range(5):
+        one, _, _ = ctx.propagate_fixed(0, batch[k], h, 200)
+        assert np.array_equal(one, got[k])
+
+
+def adaptive_oracle(dom, A, fib, method, tol, traces=1):
+    f = O.Fibre(length=fib["length"], beta=fib["beta"], gamma=fib["gamma"], method=method, local_error=tol,
+                traces=traces, self_steepening=fib.get("self_steepening", False), use_all=fib.get("use_all", False))
+    out = f(dom, A.copy())
+    return out, f
+
+
+def adaptive_params(lib, fib, tol, traces=1, eta=5, total_steps=100):
+    p = lib.AdaptiveParams()
+    p.length = fib["length"]
+    h = fib["length"] / traces if traces > total_steps else fib["length"] / total_steps
+    p.h0 = min(h, 1e-6)
+    p.local_error = tol
+    p.safety, p.max_factor, p.min_factor, p.step_size_min = 0.9, 10.0, 0.2, 1e-37
+    p.traces, p.total_attempts, p.steps_max, p.eta = traces, 1000, 100000, eta
+    return p
+
+
+@pytest.mark.parametrize("N,length", [(1024, 1.0e-5), (8192, 1.0e-5)])
+def test_c2_adaptive_short(lib, N, length):
+    """C2 physics (Raman + self-steepening, tol 1e-6) over the first part of
+    the fibre: same attempt sequence and final field as the oracle."""
+    dom, A, fib = supercontinuum_c2(N)
+    fib["length"] = length
+    ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+    log = f.stepper.attempt_log
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6), log_cap=len(log) + 8)
+    r = out["results"][0]
+    err = rel_l2(out["A"], ref)
+    print(f"C2 N={N} L={length}: attempts {r['n_attempts']} (oracle {len(log)}), rel L2 {err:.3e}, "
+          f"kernel {ctx.last_kernel_ms():.1f} ms", flush=True)
+    assert r["status"] == 0
+    assert r["n_attempts"] == len(log)
+    ref_h = np.array([a[0] for a in log])
+    ref_acc = np.array([a[2] for a in log], dtype=np.uint8)
+    dh = np.max(np.abs(out["log_h"][0, :len(log)] / ref_h - 1))
+    print(f"  max rel step-size difference {dh:.2e}", flush=True)
+    assert np.array_equal(out["log_accept"][0, :len(log)], ref_acc)
+    assert dh < 1e-8
+    assert err <= FIELD_TOL
+
+
+def test_c2_replay_matches_oracle(lib):
+    """Replay the oracle's step sequence: isolates arithmetic from control."""
+    dom, A, fib = supercontinuum_c2(8192)
+    fib["length"] = 1.0e-5
+    ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+    log = f.stepper.attempt_log
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    rh = np.array([a[0] for a in log])
+    ra = np.array([a[2] for a in log], dtype=np.uint8)
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6), log_cap=len(log), replay=(rh, ra))
+    d_dev = out["log_delta"][0]
+    d_ref = np.array([a[1] for a in log])
+    err = rel_l2(out["A"], ref)
+    print(f"replay: rel L2 {err:.3e}, max rel delta diff {np.max(np.abs(d_dev - d_ref) / d_ref):.2e}", flush=True)
+    assert out["results"][0]["status"] == 0
+    assert np.allclose(d_dev, d_ref, rtol=1e-6)
+    assert err <= FIELD_TOL
+
+
+def test_adaptive_nan_input_raises_small_step(lib):
+    """stepper.py:234-237, 349-351: NaN fields are rejected until h < 1e-37."""
+    dom, A, fib = soliton_c1(1024)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    A = A.copy()
+    A[10] = np.nan
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6))
+    assert out["results"][0]["status"] == lib.STATUS_SMALL_STEP
+
+
+def test_adaptive_steps_max(lib):
+    dom, A, fib = soliton_c1(1024)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    p = adaptive_params(lib, fib, 1e-9)
+    p.steps_max = 3
+    out = ctx.propagate_adaptive(0, A, p)
+    assert out["results"][0]["status"] == lib.STATUS_MAX_STEPS
+    assert out["results"][0]["n_accepted"] == 2
