@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the GNLSE engine on its headline workload.
+
+Metric: GNLSE propagations/s at a 2^13 grid (FP64) on one GPU. A "step" is
+one batched launch that propagates B independent realisations of the
+supercontinuum workload (8192 samples over 12.5 ps at 835 nm; 50 fs FWHM
+sech at 10 kW with one-photon-per-mode input noise seeded per realisation;
+15 cm of photonic crystal fibre, gamma = 110 /W/km, beta2..beta10,
+self-steepening and Blow-Wood Raman; adaptive RK4IP with step doubling,
+local error 1e-6) through the whole fibre. Inputs are resident in HBM when
+the timed region starts; every instance runs the full adaptive propagation.
+
+Printed: one JSON line with the metric, the roofline of the propagation
+kernel (algorithmic bytes = 72 FFTs x 32 B x N per attempted step, over the
+kernel's HIP-event duration, against 8 TB/s HBM) and a CPU baseline (the
+NumPy oracle on one host core, timed on a bounded sample of the same
+workload).
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "GNLSE propagations/s at 2^13 grid (FP64) on 1 GPU; surrogate speedup at 1% err"
+HBM_PEAK_GBS = 8000.0
+FFTS_PER_ATTEMPT = 72  # 3 RK4IP calls x (4 linear x 2 + 4 Raman/SS nonlinear x 4) FFTs
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "supercontinuum_8192_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0, help="realisations per launch (0: one per CU)")
+    ap.add_argument("--length", type=float, default=1.5e-4, help="fibre length in km (default: the full 15 cm)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cu_count():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            return torch.cuda.get_device_properties(0).multi_processor_count
+    except Exception:
+        pass
+    return 256
+
+
+def cpu_baseline(A0, length, attempts_full, budget_s):
+    """The oracle (NumPy restatement of pyofss) on one host core, on the first
+    attempted steps of the same propagation; scaled by the number of attempted
+    steps the full propagation takes."""
+    from threadpoolctl import threadpool_limits
+
+    import oracle as O
+    from pyofsschanged_amd.workloads import DUDLEY_BETA, C_NM_PS
+
+    with threadpool_limits(1):
+        dom = O.Domain(1, 8192, 12.5, C_NM_PS / 835.0)
+
+        def run(n):
+            f = O.Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
+                        method="ark4ip", local_error=1e-6)
+            f.linearity(dom)
+            f.nonlinearity(dom)
+            f.stepper.storage(dom)
+            f.stepper.storage.reset_fft_counter()
+            t = time.perf_counter()
+            f.stepper.adaptive_stepper(A0.copy(), None, max_attempts=n)
+            return time.perf_counter() - t, len(f.stepper.attempt_log)
+
+        t_probe, n_probe = run(8)
+        n = max(8, int(budget_s / (t_probe / max(n_probe, 1))))
+        elapsed, done = run(n)
+    per_attempt = elapsed / done
+    return {
+        "value": 1.0 / (per_attempt * attempts_full),
+        "unit": "propagations/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"NumPy oracle, first {done} attempted steps of one realisation in {elapsed:.1f} s "
+                   f"({per_attempt * 1e3:.2f} ms/attempt), scaled to the {attempts_full:.0f} attempted steps of "
+                   f"a full propagation"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances shard by realisation, "
+              f"with no exchange until ensemble statistics)", file=sys.stderr)
+        sys.exit(2)
+
+    from pyofsschanged_amd import workloads
+    from pyofsschanged_amd._lib import METHOD_RK4IP
+
+    domain, pulse, fibre = workloads.supercontinuum(length=args.length)
+    N = domain.total_samples
+    ctx = fibre.prepare(domain)
+    params = fibre.stepper.adaptive_params()
+    B = args.batch or cu_count()
+
+    A0 = pulse.generate(domain.t)
+    inputs = np.stack([A0 + workloads.oppm_noise(domain, seed=k) for k in range(B)]).astype(np.complex128)
+    nbytes = inputs.nbytes
+    d_in = ctx.malloc(nbytes)
+    d_out = ctx.malloc(nbytes)
+    ctx.htod(d_in, inputs)
+
+    for _ in range(args.warmup):
+        ctx.propagate_adaptive_device(METHOD_RK4IP, B, params, d_in, d_out)
+        print(f"warmup launch {ctx.last_kernel_ms():.1f} ms", file=sys.stderr, flush=True)
+
+    kernel_ms = []
+    attempts = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res = ctx.propagate_adaptive_device(METHOD_RK4IP, B, params, d_in, d_out)
+        kernel_ms.append(ctx.last_kernel_ms())
+        attempts.append(sum(r["n_attempts"] for r in res))
+        bad = [r["status"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError(f"{len(bad)} instances failed with status {set(bad)}")
+        print(f"step {k}: {kernel_ms[-1]:.1f} ms, {attempts[-1] / B:.1f} attempts/instance", file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t0
+    out = np.empty_like(inputs)
+    ctx.dtoh(out, d_out)
+    ctx.free(d_in)
+    ctx.free(d_out)
+    if not np.all(np.isfinite(out)):
+        raise RuntimeError("non-finite output field")
+
+    ms_per_step = 1e3 * wall / args.steps
+    value = B * args.steps / wall
+    mean_kernel_s = 1e-3 * float(np.mean(kernel_ms))
+    bytes_per_launch = float(np.mean(attempts)) * FFTS_PER_ATTEMPT * 32 * N
+    achieved = bytes_per_launch / mean_kernel_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+        if t.get("batch") == B and abs(t.get("length_km", 0) - args.length) < 1e-12:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "propagations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("supercontinuum: 8192 samples / 12.5 ps at 835 nm, 50 fs FWHM sech 10 kW + one-photon-"
+                         "per-mode noise, 15 cm PCF gamma 110 /W/km beta2..beta10, self-steepening + Blow-Wood "
+                         "Raman, adaptive RK4IP local error 1e-6"),
+            "grid": N,
+            "batch": B,
+            "length_km": args.length,
+            "mean_attempts_per_propagation": float(np.mean(attempts)) / B,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(inputs[0], args.length, float(np.mean(attempts)) / B, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

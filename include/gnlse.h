@@ -134,15 +134,18 @@ int gnlse_apply_nonlinear(gnlse_ctx* ctx, int64_t batch, const double* in, doubl
 int gnlse_apply_exp_nonlinear(gnlse_ctx* ctx, double h, int64_t batch, const double* A, const double* B,
                               double* out);
 
-/* One integrator call Solver.<method>(A, z, h) with exp(factor*h/2)
- * recomputed (solver.py:266-275, 339-354 through linearity.py:241-256). */
-int gnlse_step(gnlse_ctx* ctx, int method, double h, int64_t batch, const double* in, double* out);
+/* One integrator call Solver.<method>(A, z, h) (solver.py:266-275, 339-354)
+ * with linear sub-steps exp(factor*h_exp) (linearity.py:241-272): h_exp <= 0
+ * means h/2 (recomputed, as default_exp_f); a Linearity that caches passes
+ * the step of its first call (default_exp_f_cached). */
+int gnlse_step(gnlse_ctx* ctx, int method, double h, double h_exp, int64_t batch, const double* in, double* out);
 
 /* Fixed-step propagation, Stepper.standard_stepper (stepper.py:164-222) with
- * the cached linear factor of linearity.py:220-224/258-272. Traces: after
+ * the cached linear factor exp(factor*h_exp) of linearity.py:220-224/258-272
+ * (h_exp <= 0 means h/2, the step of the first linear call). Traces: after
  * step i when trace_every > 0 and i % trace_every == 0, as stepper.py:206-209;
  * traces is complex128[batch][max_traces][n]. n_traces: int64[batch]. */
-int gnlse_propagate_fixed(gnlse_ctx* ctx, int method, int64_t batch, double h, int64_t nsteps,
+int gnlse_propagate_fixed(gnlse_ctx* ctx, int method, int64_t batch, double h, double h_exp, int64_t nsteps,
                           int64_t trace_every, int64_t max_traces, const double* A_in, double* A_out,
                           double* traces, int64_t* n_traces);
 

@@ -1017,9 +1017,11 @@ class Stepper:
             h = 1e-6
         return h
 
-    def adaptive_stepper(self, A, refrence_length, replay=None):
+    def adaptive_stepper(self, A, refrence_length, replay=None, max_attempts=None):
         """stepper.py:247-376. ``replay`` (optional) is a list of step sizes
-        to use for successive attempts instead of the controller's."""
+        to use for successive attempts instead of the controller's;
+        ``max_attempts`` (optional) stops the run early and returns the field
+        reached, for timing a bounded sample."""
         self.A_out = A
         z = 0.0
         h = self.initial_step()
@@ -1035,6 +1037,9 @@ class Stepper:
             if (z + h) > self.length:
                 h = self.length - z
             for ta in range(0, self.total_attempts):
+                if max_attempts is not None and n_attempt >= max_attempts:
+                    self.storage.store_current_fft_count()
+                    return self.A_out
                 if replay is not None:
                     h = replay[n_attempt]
                 n_attempt += 1

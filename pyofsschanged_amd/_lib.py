@@ -89,8 +89,8 @@ SIGNATURES = {
     "gnlse_apply_linear_factor": [_P, _I64, _P, _P],
     "gnlse_apply_nonlinear": [_P, _I64, _P, _P],
     "gnlse_apply_exp_nonlinear": [_P, _D, _I64, _P, _P, _P],
-    "gnlse_step": [_P, _INT, _D, _I64, _P, _P],
-    "gnlse_propagate_fixed": [_P, _INT, _I64, _D, _I64, _I64, _I64, _P, _P, _P, _P],
+    "gnlse_step": [_P, _INT, _D, _D, _I64, _P, _P],
+    "gnlse_propagate_fixed": [_P, _INT, _I64, _D, _D, _I64, _I64, _I64, _P, _P, _P, _P],
     "gnlse_propagate_adaptive": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _I64, _P, _P, _I64, _P,
                                  _P, _P, _P, _I64, _P, _P, _P, _P],
     "gnlse_malloc": [_P, _I64, ctypes.POINTER(_P)],
@@ -251,20 +251,22 @@ class Context:
         self._call("gnlse_apply_exp_nonlinear", float(h), a.shape[0], _ptr(a), _ptr(b), _ptr(out))
         return out.reshape(shape)
 
-    def step(self, method, A, h):
+    def step(self, method, A, h, h_exp=None):
+        """One integrator call; linear sub-steps use exp(factor*h_exp)
+        (default h/2)."""
         a, shape = self._batched(A)
         out = np.empty_like(a)
-        self._call("gnlse_step", int(method), float(h), a.shape[0], _ptr(a), _ptr(out))
+        self._call("gnlse_step", int(method), float(h), float(h_exp or 0.0), a.shape[0], _ptr(a), _ptr(out))
         return out.reshape(shape)
 
     # -- propagation ------------------------------------------------------------
-    def propagate_fixed(self, method, A, h, nsteps, trace_every=0, max_traces=0):
+    def propagate_fixed(self, method, A, h, nsteps, trace_every=0, max_traces=0, h_exp=None):
         a, shape = self._batched(A)
         B = a.shape[0]
         out = np.empty_like(a)
         traces = np.empty((B, max(max_traces, 0), self.n), np.complex128)
         ntr = np.zeros(B, np.int64)
-        self._call("gnlse_propagate_fixed", int(method), B, float(h), int(nsteps), int(trace_every),
+        self._call("gnlse_propagate_fixed", int(method), B, float(h), float(h_exp or 0.0), int(nsteps), int(trace_every),
                    int(max_traces), _ptr(a), _ptr(out), _ptr(traces) if max_traces > 0 else None,
                    _ptr(ntr) if max_traces > 0 else None)
         return out.reshape(shape), traces, ntr

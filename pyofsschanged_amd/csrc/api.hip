@@ -347,7 +347,7 @@ int gnlse_apply_exp_nonlinear(gnlse_ctx* c, double h, int64_t batch, const doubl
   return io_finish(c, batch, out);
 }
 
-int gnlse_step(gnlse_ctx* c, int method, double h, int64_t batch, const double* in, double* out) {
+int gnlse_step(gnlse_ctx* c, int method, double h, double h_exp, int64_t batch, const double* in, double* out) {
   RET_IF(check_ready(c, true, true));
   if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
   HIPCHK(hipSetDevice(c->device));
@@ -355,13 +355,13 @@ int gnlse_step(gnlse_ctx* c, int method, double h, int64_t batch, const double* 
   RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
-  RET_IF(timed_end(c, c->tab->step(c->stream, c->ops, method, h, batch, (const double2*)c->io_in.p,
+  RET_IF(timed_end(c, c->tab->step(c->stream, c->ops, method, h, h_exp > 0.0 ? h_exp : 0.5 * h, batch, (const double2*)c->io_in.p,
                                    (double2*)c->io_out.p, (double2*)c->ws.p)));
   return io_finish(c, batch, out);
 }
 
-int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, int64_t nsteps, int64_t trace_every,
-                          int64_t max_traces, const double* A_in, double* A_out, double* traces,
+int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, double h_exp, int64_t nsteps,
+                          int64_t trace_every, int64_t max_traces, const double* A_in, double* A_out, double* traces,
                           int64_t* n_traces) {
   RET_IF(check_ready(c, true, true));
   if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
@@ -369,8 +369,9 @@ int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, int
   if (max_traces > 0 && (!traces || !n_traces)) return fail(GNLSE_ERR_ARG, "null trace outputs");
   HIPCHK(hipSetDevice(c->device));
   const int64_t n = c->n;
-  // linearity.py:220-224, 258-262: the cached factor is exp(factor*h/2)
-  const double hh = 0.5 * h;
+  // linearity.py:220-224, 258-262: the cached factor exp(factor*h_exp), where
+  // h_exp is the step of the first linear call (h/2 for both integrators)
+  const double hh = h_exp > 0.0 ? h_exp : 0.5 * h;
   if (!(c->efix_h == hh)) {
     hipError_t e = c->tab->make_exp(c->stream, c->ops, hh, (double2*)c->efix.p);
     if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("make_exp: ") + hipGetErrorString(e));

@@ -48,7 +48,7 @@ struct LaunchTable {
   hipError_t (*nonlin)(hipStream_t, const Ops&, long long batch, const double2* in, double2* out, double2* ws);
   hipError_t (*exp_nonlin)(hipStream_t, const Ops&, double h, long long batch, const double2* A, const double2* B,
                            double2* out, double2* ws);
-  hipError_t (*step)(hipStream_t, const Ops&, int method, double h, long long batch, const double2* in,
+  hipError_t (*step)(hipStream_t, const Ops&, int method, double h, double h_exp, long long batch, const double2* in,
                      double2* out, double2* ws);
   hipError_t (*fixed)(hipStream_t, const Ops&, int method, const FixedParams&, long long batch, const double2* in,
                       double2* out, double2* ws);
@@ -139,13 +139,13 @@ __global__ void __launch_bounds__(T) k_exp_nonlin(Ops ops, double h, const doubl
 
 template <int N, int T, int METHOD, int K>
 __global__ void __launch_bounds__(T)
-    k_step(Ops ops, double h, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
+    k_step(Ops ops, double h, double h_exp, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
   GN_SHARED(N, T, K);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
   double2 x[P];
-  d.make_exp(ws + S_EH * N, 0.5 * h);
+  d.make_exp(ws + S_EH * N, h_exp);
   d.load(x, in + b * N);
   d.store(ws + S_A * N, x);
   integrator<METHOD>(d, x, ws + S_A * N, ws + S_EH * N, h, ws);
@@ -356,10 +356,10 @@ struct Launcher {
     GN_BY_KIND(o, k_exp_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, h, A, B, out, ws);
     return hipGetLastError();
   }
-  static hipError_t step(hipStream_t s, const Ops& o, int method, double h, long long batch, const double2* in,
+  static hipError_t step(hipStream_t s, const Ops& o, int method, double h, double h_exp, long long batch, const double2* in,
                          double2* out, double2* ws) {
-    if (method == M_RK4IP) GN_BY_KIND(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
-    else GN_BY_KIND(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    if (method == M_RK4IP) GN_BY_KIND(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    else GN_BY_KIND(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t fixed(hipStream_t s, const Ops& o, int method, const FixedParams& p, long long batch,

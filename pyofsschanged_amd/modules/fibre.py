@@ -86,20 +86,37 @@ class Fibre:
             self.nonlinearity.bind(self._ctx)
         return self._ctx
 
+    def linear_step(self, h):
+        """Step of exp(factor*h) used by a linear sub-step of size h: with a
+        caching Linearity (fixed-step methods) the step of its first call,
+        as linearity.py:258-262 caches exp(factor*h) once."""
+        lin = self.linearity
+        if not lin.use_cache:
+            return h
+        if lin.cached_factor is None:
+            lin.cache(h)
+        return lin.cached_factor
+
     def _count_ffts(self, method, calls):
         """Advance field.fft_counter as pyofss counts FFTs (field.py:118-134)."""
         nl = _NL_FFTS[self.nonlinearity.kind]
         per_call = 8 + 4 * nl if method == _lib.METHOD_RK4IP else 4 + nl
         field.fft_counter += per_call * calls
 
-    def __call__(self, domain, field_in):
-        """fibre.py:136-148"""
+    def prepare(self, domain):
+        """Operator set-up of fibre.py:137-143: build the linear and nonlinear
+        operators for ``domain`` and load them into the device context."""
         if self.domain != domain or self.linearity.factor is None or self.nonlinearity.factor is None:
             self._device_context()
             self.linearity(domain)
             self.nonlinearity(domain)
             self.stepper.storage(domain)
             self.domain = domain
+        return self._device_context()
+
+    def __call__(self, domain, field_in):
+        """fibre.py:136-148"""
+        self.prepare(domain)
         self.calculate_reference_length(field_in)
         return self.stepper(field_in, self.reference_length)
 

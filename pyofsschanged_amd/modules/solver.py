@@ -70,11 +70,11 @@ class Solver:
         """solver.py:339-354 on the GPU."""
         fibre = _engine_of(f)
         fibre._count_ffts(_lib.METHOD_RK4IP, 1)
-        return fibre._device_context().step(_lib.METHOD_RK4IP, A, h)
+        return fibre._device_context().step(_lib.METHOD_RK4IP, A, h, fibre.linear_step(0.5 * h))
 
     @staticmethod
     def ss_symmetric(A, z, h, f):
         """solver.py:266-275 on the GPU."""
         fibre = _engine_of(f)
         fibre._count_ffts(_lib.METHOD_SS_SYMMETRIC, 1)
-        return fibre._device_context().step(_lib.METHOD_SS_SYMMETRIC, A, h)
+        return fibre._device_context().step(_lib.METHOD_SS_SYMMETRIC, A, h, fibre.linear_step(0.5 * h))

@@ -139,7 +139,8 @@ class Stepper:
         trace_every = int(self.total_steps / self.traces) if steps else 0
         ctx = fibre._device_context()
         A_out, tr, ntr = ctx.propagate_fixed(self.solver.method_id, np.asarray(A, dtype=np.complex128), h,
-                                             self.total_steps, trace_every, len(steps))
+                                             self.total_steps, trace_every, len(steps),
+                                             h_exp=fibre.linear_step(0.5 * h))
         fibre._count_ffts(self.solver.method_id, self.total_steps)
         for k, i in enumerate(steps):
             self.storage.append(zs[i] + h, tr[0, k].copy())

@@ -1,0 +1,58 @@
+"""Synthetic workloads used by the benchmark and the smoke run, in pyofss units
+(length km, beta_n ps^n/km, gamma 1/(W km), power W, time ps).
+
+  soliton       second-order soliton at 1550 nm, 4096 samples over 100 ps,
+                T0 = 1 ps, beta2 = -20 ps^2/km, gamma = 1.3 /W/km, one soliton
+                period (pi/40 km), fixed steps.
+  supercontinuum  50 fs FWHM sech at 10 kW and 835 nm in 15 cm of photonic
+                crystal fibre (gamma = 110 /W/km, beta2..beta10 of
+                examples/dudley_sc/fig_3.py:35-36), self-steepening and
+                Blow-Wood Raman, 8192 samples over 12.5 ps, adaptive RK4IP with
+                local error 1e-6.
+
+Input noise: one photon per mode with a uniform random spectral phase,
+|A~_k| = sqrt(h nu_k / (N dt)) (h = 6.626e-10 W ps^2, the constant of
+pyofss/modules/linearity.py:290), seeded per realisation with numpy's
+Philox(key=seed).
+"""
+
+import math
+
+import numpy as np
+
+from .domain import Domain
+from .field import fftshift, ifft
+from .modules.fibre import Fibre
+from .modules.sech import Sech
+
+C_NM_PS = Domain.vacuum_light_speed
+PLANCK_W_PS2 = 6.626e-10
+
+DUDLEY_BETA = [0.0, 0.0, -11.830, 8.1038e-2, -9.5205e-5, 2.0737e-7, -5.3943e-10, 1.3486e-12, -2.5495e-15,
+               3.0524e-18, -1.7140e-21]
+
+
+def oppm_noise(domain, seed):
+    """One-photon-per-mode noise field in the temporal domain."""
+    rng = np.random.Generator(np.random.Philox(key=seed))
+    N = domain.total_samples
+    nu = fftshift(domain.nu)
+    amp = np.sqrt(PLANCK_W_PS2 * nu / (N * domain.dt))
+    spec = amp * np.exp(1j * rng.uniform(0.0, 2.0 * np.pi, N))
+    return ifft(spec)
+
+
+def soliton(samples=4096, method="rk4ip", total_steps=1000, traces=50):
+    domain = Domain(1, samples, 100.0, C_NM_PS / 1550.0)
+    pulse = Sech(peak_power=4 * 20.0 / 1.3, width=1.0)
+    fibre = Fibre(length=math.pi / 40, beta=[0.0, 0.0, -20.0], gamma=1.3, method=method, total_steps=total_steps,
+                  traces=traces)
+    return domain, pulse, fibre
+
+
+def supercontinuum(samples=8192, length=1.5e-4, local_error=1e-6, traces=1):
+    domain = Domain(1, samples, 12.5, C_NM_PS / 835.0)
+    pulse = Sech(peak_power=1e4, width=0.050, using_fwhm=True)
+    fibre = Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
+                  method="ark4ip", local_error=local_error, traces=traces)
+    return domain, pulse, fibre

@@ -87,7 +87,7 @@ KINDS = [dict(), dict(use_all=True), dict(use_all=True, self_steepening=True), d
 
 
 @pytest.mark.parametrize("flags", KINDS, ids=lambda d: "+".join(sorted(d)) or "kerr")
-@pytest.mark.parametrize("N", [1024, 8192])
+@pytest.mark.parametrize("N", [512, 1024, 2048, 4096, 8192])
 def test_nonlinear_operator(lib, flags, N):
     dom, A, fib = supercontinuum_c2(N)
     lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], **flags)

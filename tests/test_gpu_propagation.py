@@ -3,10 +3,11 @@
 Tolerance: relative L2 field error <= 1e-9 against the oracle's NumPy
 complex128 path on identical inputs, for every returned field. Adaptive runs
 must also reproduce the oracle's attempt sequence: identical accept/reject
-decisions and attempt count, step sizes to 1e-8 relative. The controller's
-error estimate is the norm of A_fine - A_coarse, a difference of two fields
-that agree to ~1e-6, so it carries ~1e-10 relative roundoff in any FP64
-implementation; h depends on it through its fifth root.
+decisions and attempt count, step sizes to 1e-6 relative. The controller's
+error estimate delta is the norm of A_fine - A_coarse, a difference of two
+fields that agree to ~delta ~ 1e-6, so two FP64 implementations whose fields
+differ by e (~1e-12 relative) give deltas that differ by ~e/delta (~1e-6);
+h depends on delta through its fifth root.
 """
 
 import math
@@ -109,7 +110,7 @@ def test_c2_adaptive_short(lib, N, length):
     dh = np.max(np.abs(out["log_h"][0, :len(log)] / ref_h - 1))
     print(f"  max rel step-size difference {dh:.2e}", flush=True)
     assert np.array_equal(out["log_accept"][0, :len(log)], ref_acc)
-    assert dh < 1e-8
+    assert dh < 1e-6
     assert err <= FIELD_TOL
 
 
