@@ -43,7 +43,22 @@ def parse():
     ap.add_argument("--length", type=float, default=1.5e-4, help="fibre length in km (default: the full 15 cm)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: FETCH_SIZE/WRITE_SIZE of this
+    dispatch calibrate the counters read on the propagation kernel."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
 
 
 def cu_count():
@@ -96,6 +111,9 @@ def cpu_baseline(A0, length, attempts_full, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances shard by realisation, "
               f"with no exchange until ensemble statistics)", file=sys.stderr)
@@ -146,12 +164,14 @@ def main():
     mean_kernel_s = 1e-3 * float(np.mean(kernel_ms))
     bytes_per_launch = float(np.mean(attempts)) * FFTS_PER_ATTEMPT * 32 * N
     achieved = bytes_per_launch / mean_kernel_s / 1e9
+    # measured HBM bytes per attempted step (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    # copy-calibrated, same kernel and batch) times this launch's attempts
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
-        if t.get("batch") == B and abs(t.get("length_km", 0) - args.length) < 1e-12:
-            traffic = t.get("hbm_bytes_per_launch")
+        if t.get("batch", B) == B and "hbm_bytes_per_attempt" in t:
+            traffic = t["hbm_bytes_per_attempt"] * float(np.mean(attempts))
 
     line = {
         "metric": METRIC,
