@@ -12,7 +12,8 @@ import threading
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgnlse_hip.so")
+LIB_PATH = os.environ.get("GNLSE_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                             "libgnlse_hip.so")
 
 METHOD_RK4IP = 0
 METHOD_SS_SYMMETRIC = 1

@@ -33,20 +33,23 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src, extra):
-    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
+def _compile(src, extra, objdir=OBJ):
+    obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
     cmd = [HIPCC, *CFLAGS, *extra, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r.returncode, r.stdout + r.stderr
 
 
-def build(jobs=None, extra=(), verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(jobs=None, extra=(), verbose=False, out=LIB, objdir=OBJ):
+    """Compile every csrc/*.hip for gfx950 and link ``out``. ``extra`` holds
+    additional hipcc flags, e.g. -DGN_T_8192=1024 to pick another workgroup
+    size for a grid."""
+    os.makedirs(objdir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), os.cpu_count() or 4, 16)
     objs = []
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        for src, obj, rc, log in ex.map(lambda s: _compile(s, list(extra)), srcs):
+        for src, obj, rc, log in ex.map(lambda s: _compile(s, list(extra), objdir), srcs):
             # the host pass does not know the packed-fp32 feature; that warning is expected
             lines = [l for l in log.splitlines() if "packed-fp32-ops" not in l]
             if rc != 0:
@@ -55,23 +58,27 @@ def build(jobs=None, extra=(), verbose=False):
             if verbose and lines:
                 sys.stderr.write("\n".join(lines) + "\n")
             objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("link of libgnlse_hip.so failed")
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError(f"link of {out} failed")
+    os.replace(tmp, out)
+    return out
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--resource-usage", action="store_true", help="print per-kernel register/LDS usage")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor define")
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("--objdir", default=OBJ)
     a = ap.parse_args(argv)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if a.resource_usage else []
-    print(build(a.jobs, extra, verbose=a.resource_usage))
+    extra += ["-D" + d for d in a.defines]
+    print(build(a.jobs, extra, verbose=a.resource_usage, out=a.out, objdir=a.objdir))
 
 
 if __name__ == "__main__":

@@ -1,3 +1,6 @@
-// Kernel instantiations for N = 1024 samples, 128 threads per instance.
+// Kernel instantiations for N = 1024 samples; GN_T_1024 threads per instance.
 #include "kernels.hpp"
-GN_DEFINE_GRID(1024, 128)
+#ifndef GN_T_1024
+#define GN_T_1024 128
+#endif
+GN_DEFINE_GRID(1024, GN_T_1024)

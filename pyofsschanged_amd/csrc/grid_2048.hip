@@ -1,3 +1,6 @@
-// Kernel instantiations for N = 2048 samples, 256 threads per instance.
+// Kernel instantiations for N = 2048 samples; GN_T_2048 threads per instance.
 #include "kernels.hpp"
-GN_DEFINE_GRID(2048, 256)
+#ifndef GN_T_2048
+#define GN_T_2048 256
+#endif
+GN_DEFINE_GRID(2048, GN_T_2048)

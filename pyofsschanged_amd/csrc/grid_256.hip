@@ -1,3 +1,6 @@
-// Kernel instantiations for N = 256 samples, 64 threads per instance.
+// Kernel instantiations for N = 256 samples; GN_T_256 threads per instance.
 #include "kernels.hpp"
-GN_DEFINE_GRID(256, 64)
+#ifndef GN_T_256
+#define GN_T_256 64
+#endif
+GN_DEFINE_GRID(256, GN_T_256)

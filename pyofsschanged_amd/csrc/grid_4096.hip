@@ -1,3 +1,6 @@
-// Kernel instantiations for N = 4096 samples, 512 threads per instance.
+// Kernel instantiations for N = 4096 samples; GN_T_4096 threads per instance.
 #include "kernels.hpp"
-GN_DEFINE_GRID(4096, 512)
+#ifndef GN_T_4096
+#define GN_T_4096 512
+#endif
+GN_DEFINE_GRID(4096, GN_T_4096)

@@ -1,3 +1,6 @@
-// Kernel instantiations for N = 512 samples, 64 threads per instance.
+// Kernel instantiations for N = 512 samples; GN_T_512 threads per instance.
 #include "kernels.hpp"
-GN_DEFINE_GRID(512, 64)
+#ifndef GN_T_512
+#define GN_T_512 64
+#endif
+GN_DEFINE_GRID(512, GN_T_512)

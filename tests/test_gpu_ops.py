@@ -7,9 +7,10 @@ Tolerances (relative L2 unless stated):
     where the spectrum of A is at the FFT roundoff floor (~1e-16 of its
     peak); both implementations amplify that floor by |D|, so the result is
     only conditioned to ~1e-16 * max|D| / ||D A~||.
-  * exp_non with self-steepening: 1e-10. nonlinearity.py:230-264 divide by B,
-    which reaches the underflow region in the pulse tails; the quotient is
-    conditioned by 1/|B| there in both implementations.
+  * exp_non: 1e-12 on a field with a weak CW background. nonlinearity.py:
+    216, 233, 261 divide by B; on a bare sech the tails reach the underflow
+    region, where the quotient (and the reference's own result) is rounding
+    noise in any implementation.
 """
 
 import numpy as np
@@ -97,13 +98,16 @@ def test_nonlinear_operator(lib, flags, N):
     err = rel_l2(got, ref)
     print(f"N={N} {flags} non err {err:.2e}", flush=True)
     assert err < OP_TOL
-    B = lin.default_exp_f(A, 2e-7)
-    got = ctx.apply_exp_nonlinear(A, 2e-7, B)
-    with np.errstate(all="ignore"):
-        ref = nl.exp_non(A, 2e-7, B)
+    # exp_non divides by B (nonlinearity.py:216, 233, 261); on a weak CW
+    # background the quotient stays well conditioned (bare sech tails reach
+    # underflow, where both implementations return rounding noise)
+    Ab = A + 1e-2 * np.sqrt(1e4)
+    B = lin.default_exp_f(Ab, 2e-7)
+    got = ctx.apply_exp_nonlinear(Ab, 2e-7, B)
+    ref = nl.exp_non(Ab, 2e-7, B)
     err = rel_l2(got, ref)
     print(f"N={N} {flags} exp_non err {err:.2e}", flush=True)
-    assert err < (1e-10 if flags.get("self_steepening") else OP_TOL)
+    assert err < OP_TOL
 
 
 @pytest.mark.parametrize("method", ["rk4ip", "ss_symmetric"])
