@@ -16,9 +16,11 @@ const LaunchTable* table_1024();
 const LaunchTable* table_2048();
 const LaunchTable* table_4096();
 const LaunchTable* table_8192();
+const LaunchTable* table_16384();
 
 const LaunchTable* find_table(long long n) {
-  const LaunchTable* all[] = {table_256(), table_512(), table_1024(), table_2048(), table_4096(), table_8192()};
+  const LaunchTable* all[] = {table_256(), table_512(), table_1024(), table_2048(), table_4096(), table_8192(),
+                               table_16384()};
   for (const LaunchTable* t : all)
     if (t->n == n) return t;
   return nullptr;

@@ -15,14 +15,15 @@
 // -i transform is conj(F+(conj(x))), which is exact. One transform body per
 // kernel keeps code size and register pressure down.
 //
-// LDS layout: element e is stored at e + (e >> log2(P)). The first pass
-// writes with a stride of P elements per lane; the padding makes that stride
-// P + 1 (odd), so consecutive lanes land in distinct banks. Reads of owned
-// positions are unit-stride across lanes. When the padded field does not fit
-// the 160 KiB LDS of a gfx950 CU, the exchange moves real parts and then
-// imaginary parts through a half-size buffer (SPLIT mode).
+// LDS layout: element e is stored at e + (e >> log2(R)), R the largest
+// radix. The first pass writes with a stride of R elements per lane; the
+// padding makes that stride R + 1 (odd), so consecutive lanes land in
+// distinct banks. Reads of owned positions are unit-stride across lanes.
+// When the padded field does not fit the 160 KiB LDS of a gfx950 CU, the
+// exchange moves real parts and then imaginary parts through a half-size
+// buffer (SPLIT mode).
 //
-// Twiddles: each thread keeps one exact base per pass (from a table tw[m] =
+// Twiddles: each thread keeps a few exact bases per pass (from a table tw[m] =
 // exp(+2 pi i m / N) built on the host in extended precision) and forms the
 // powers it needs in registers, so a pass issues no memory loads for them.
 #pragma once
@@ -126,26 +127,29 @@ __device__ __forceinline__ double2 opaque(double2 w) {
   return w;
 }
 
-// Pass plan: radix P for every pass except one remainder pass, placed second
-// to last (first when there are only two passes). Every radix divides P.
+// Pass plan: radix RM = min(P, 16) for every pass except one remainder pass,
+// placed second to last (first when there are only two passes). Every radix
+// divides P, so a thread runs P/R butterflies per pass.
 struct Plan {
   int npass;
   int radix[8];
 };
+constexpr int max_radix(int P) { return P < 16 ? P : 16; }
 constexpr Plan make_plan(int N, int P) {
+  const int RM = max_radix(P);
   Plan p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
   int m = 0, n = N;
-  while (n % P == 0 && n > 1) {
-    n /= P;
+  while (n % RM == 0 && n > 1) {
+    n /= RM;
     ++m;
   }
   if (n == 1) {
     p.npass = m;
-    for (int i = 0; i < m; ++i) p.radix[i] = P;
+    for (int i = 0; i < m; ++i) p.radix[i] = RM;
   } else {
     p.npass = m + 1;
     const int at = m > 1 ? m - 1 : 0;
-    for (int i = 0; i < m + 1; ++i) p.radix[i] = (i == at) ? n : P;
+    for (int i = 0; i < m + 1; ++i) p.radix[i] = (i == at) ? n : RM;
   }
   return p;
 }
@@ -159,21 +163,33 @@ constexpr int plan_ns(int N, int P, int pass) {
 template <int N, int T>
 struct FFT {
   static constexpr int P = N / T;
-  static constexpr int LP = ilog2(P);
+  // padding period: the largest radix, i.e. the write stride of a pass
+  static constexpr int LP = ilog2(max_radix(P));
   static constexpr int PADDED = N + (N >> LP);
   static constexpr bool SPLIT = (size_t)PADDED * 16 > 160 * 1024;
   static constexpr int LDS_DOUBLES = SPLIT ? PADDED : 2 * PADDED;
   static constexpr Plan plan = make_plan(N, P);
   static constexpr int NPASS = plan.npass;
-  static_assert(N % T == 0 && P >= 2 && P <= 16 && (T % 64) == 0, "bad shape");
-  static_assert(plan_ns(N, P, NPASS - 1) <= T, "every pass must see one twiddle base per thread");
+  static_assert(N % T == 0 && P >= 2 && P <= 32 && (T % 64) == 0, "bad shape");
 
-  // Per-thread twiddle bases: in pass p (NS = product of earlier radices) the
-  // butterflies of thread tid all have j mod NS = tid mod NS (T is a multiple
-  // of NS), so their twiddles are w^r with one base w = tw[(tid mod NS) *
-  // N/(NS*R)]. The bases are loaded once per kernel and kept in registers.
+  // Per-thread twiddle bases. In pass p (NS = product of earlier radices)
+  // butterfly j = tid + T*q needs the powers w^r of w = tw[(j mod NS) *
+  // N/(NS*R)]. When NS divides T, j mod NS = tid mod NS for every q and one
+  // base serves all of the thread's butterflies; otherwise (only the last
+  // pass, NS = N/R > T) there are NS/T distinct bases, indexed by q mod
+  // (NS/T). The bases are loaded once per kernel and kept in registers.
+  static constexpr int nbase(int p) {
+    const int ns = plan_ns(N, P, p);
+    return ns <= T ? 1 : ns / T;
+  }
+  static constexpr int base_off(int p) {
+    int o = 0;
+    for (int i = 0; i < p; ++i) o += nbase(i);
+    return o;
+  }
+  static constexpr int NBASE = base_off(NPASS);
   struct Twiddles {
-    double2 w[NPASS];
+    double2 w[NBASE];
   };
   __device__ static __forceinline__ void load_twiddles(Twiddles& t, const double2* __restrict__ tw) {
     const int tid = threadIdx.x;
@@ -181,7 +197,8 @@ struct FFT {
     for (int p = 0; p < NPASS; ++p) {
       const int ns = plan_ns(N, P, p);
       const int R = plan.radix[p];
-      t.w[p] = tw[(tid & (ns - 1)) * (N / (ns * R))];
+#pragma unroll
+      for (int b = 0; b < nbase(p); ++b) t.w[base_off(p) + b] = tw[((tid + T * b) & (ns - 1)) * (N / (ns * R))];
     }
   }
 
@@ -265,7 +282,7 @@ struct FFT {
       double2 v[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = x[q + r * NB];
-      if constexpr (NS > 1) twiddle<R>(v, opaque(tw.w[PASS]));
+      if constexpr (NS > 1) twiddle<R>(v, opaque(tw.w[base_off(PASS) + q % nbase(PASS)]));
       dft<R>(v);
 #pragma unroll
       for (int r = 0; r < R; ++r) x[q + r * NB] = v[r];

@@ -38,9 +38,27 @@ def supercontinuum_c2(N=8192):
     return dom, A, fibre
 
 
+def amplifier_c4(N=16384, C=0.0):
+    """Saturable-gain amplifier at 1030 nm: 200 fs FWHM Gaussian of 1 nJ
+    (P0 = 4697.2 W) with linear chirp C, 5 m of fibre, beta2 = 0.023 ps^2/m,
+    gamma = 5e-3 /W/m, Raman (use_all), 15 dB small-signal gain (3 dB/m),
+    E_sat = 5 nJ, Lorentzian gain profile 40 nm wide (amplifier.py:110-129)."""
+    dom = O.Domain(1, N, 40.0, C_NM_PS / 1030.0)
+    A = O.Gaussian(peak_power=4697.2, width=0.2, using_fwhm=True, C=C).generate(dom.t)
+    fibre = dict(length=5e-3, beta=[0.0, 0.0, 23.0], gamma=5.0, use_all=True,
+                 amplifier=dict(gain=15.0, E_sat=5.0, lamb0=1030.0, bandwidth=40.0))
+    return dom, A, fibre
+
+
+def oracle_amplifier(dom, length, gain, E_sat, lamb0=None, bandwidth=None):
+    amp = O.Amplifier(gain=gain, E_sat=E_sat, length=length, lamb0=lamb0, bandwidth=bandwidth)
+    amp.set_domain(dom)
+    return amp
+
+
 def oracle_ops(dom, beta=None, alpha=None, gamma=None, self_steepening=False, raman_scattering=False, use_all=False,
-               rs_factor=3e-3, f_R=0.18, tau_1=12.2e-3, tau_2=32e-3):
-    lin = O.Linearity(alpha=alpha, beta=beta)
+               rs_factor=3e-3, f_R=0.18, tau_1=12.2e-3, tau_2=32e-3, amplifier=None):
+    lin = O.Linearity(alpha=alpha, beta=beta, amplifier=amplifier)
     lin(dom)
     nl = O.Nonlinearity(gamma, None, self_steepening, raman_scattering, rs_factor, use_all, tau_1, tau_2, f_R)
     nl(dom)
@@ -72,4 +90,9 @@ def engine_context(dom, lin, nl, kind, amplifier=None):
         nm = nl.factor * (1.0 + nl.omega * nl.ss_factor)
         hr = nl.h_R
     ctx.set_nonlinear(kind, gamma, nl.f_R, nl.ss_factor, nl.rs_factor, nl.omega, nm, hr)
+    if amplifier is not None:
+        # energy weights of the old simps(|A|^2, t * 1e-3) rule (field.py:161-172)
+        from pyofsschanged_amd.field import simpson_weights
+        ctx.set_gain(True, float(np.log(np.power(10, 0.1 * amplifier.gain))), amplifier.length, amplifier.E_sat,
+                     amplifier.spectal_filtration_array, simpson_weights(dom.t * 1e-3))
     return ctx

@@ -32,9 +32,10 @@ def test_library_exports_every_declared_function():
     # the binding declares exactly the header's functions
     assert set(_lib.SIGNATURES) | {"gnlse_last_error"} == set(names)
     assert lib.gnlse_version() == 1
-    for n in (256, 512, 1024, 2048, 4096, 8192):
+    for n in (256, 512, 1024, 2048, 4096, 8192, 16384):
         assert _lib.grid_supported(n)
     assert not _lib.grid_supported(1000)
+    assert not _lib.grid_supported(32768)
 
 
 def test_struct_layouts_match_header():

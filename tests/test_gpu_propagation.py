@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import C_NM_PS, engine_context, nl_kind, oracle_ops, rel_l2, soliton_c1, supercontinuum_c2
+from helpers import (C_NM_PS, amplifier_c4, engine_context, nl_kind, oracle_amplifier, oracle_ops, rel_l2, soliton_c1,
+                     supercontinuum_c2)
 
 pytestmark = pytest.mark.gpu
 
@@ -71,8 +72,11 @@ def test_c1_batch_is_bitwise_consistent(lib):
 
 
 def adaptive_oracle(dom, A, fib, method, tol, traces=1):
+    amp = fib.get("amplifier") or {}
     f = O.Fibre(length=fib["length"], beta=fib["beta"], gamma=fib["gamma"], method=method, local_error=tol,
-                traces=traces, self_steepening=fib.get("self_steepening", False), use_all=fib.get("use_all", False))
+                traces=traces, self_steepening=fib.get("self_steepening", False), use_all=fib.get("use_all", False),
+                small_signal_gain=amp.get("gain"), E_sat=amp.get("E_sat"), lamb0=amp.get("lamb0"),
+                bandwidth=amp.get("bandwidth"))
     out = f(dom, A.copy())
     return out, f
 
@@ -132,6 +136,47 @@ def test_c2_replay_matches_oracle(lib):
     assert out["results"][0]["status"] == 0
     assert np.allclose(d_dev, d_ref, rtol=1e-6)
     assert err <= FIELD_TOL
+
+
+def amplifier_batch(lib, N, length, chirps):
+    """C4 runs for several chirps: oracle runs, and the engine context loaded
+    with the (chirp-independent) operators."""
+    refs, logs, inputs = [], [], []
+    for C in chirps:
+        dom, A, fib = amplifier_c4(N, C)
+        fib["length"] = length
+        ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+        refs.append(ref)
+        logs.append(f.stepper.attempt_log)
+        inputs.append(A)
+    amp = oracle_amplifier(dom, length, **fib["amplifier"])
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], use_all=True, amplifier=amp)
+    ctx = engine_context(dom, lin, nl, 1, amplifier=amp)
+    return ctx, fib, np.stack(inputs), refs, logs
+
+
+@pytest.mark.parametrize("N,length,chirps", [(2048, 5e-3, (-2.0, 0.5, 2.0)), (16384, 2e-4, (-2.0, 0.5, 2.0)),
+                                              (16384, 5e-3, (1.0,))], ids=["2048-full", "16384-short", "16384-full"])
+def test_c4_amplifier_batch(lib, N, length, chirps):
+    """C4 physics (Lorentzian saturable gain inside every linear sub-step,
+    Raman, adaptive RK4IP): chirps in one launch, each instance with its own
+    step sequence, each equal to the oracle run of its chirp. The 16384-full
+    case is the whole 5 m amplifier (about 750 attempted steps)."""
+    ctx, fib, A, refs, logs = amplifier_batch(lib, N, length, chirps)
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6), log_cap=max(len(l) for l in logs) + 8)
+    print(f"C4 N={N} L={length}: kernel {ctx.last_kernel_ms():.1f} ms", flush=True)
+    for i, C in enumerate(chirps):
+        r, log = out["results"][i], logs[i]
+        err = rel_l2(out["A"][i], refs[i])
+        ref_h = np.array([a[0] for a in log])
+        ref_acc = np.array([a[2] for a in log], dtype=np.uint8)
+        dh = np.max(np.abs(out["log_h"][i, :len(log)] / ref_h - 1))
+        print(f"  C={C}: attempts {r['n_attempts']} (oracle {len(log)}), rel L2 {err:.3e}, dh {dh:.1e}", flush=True)
+        assert r["status"] == 0
+        assert r["n_attempts"] == len(log)
+        assert np.array_equal(out["log_accept"][i, :len(log)], ref_acc)
+        assert dh < 1e-6
+        assert err <= FIELD_TOL
 
 
 def test_adaptive_nan_input_raises_small_step(lib):
