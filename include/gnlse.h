@@ -175,6 +175,33 @@ int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, c
  * the context's stream around the launch. */
 int gnlse_last_kernel_ms(gnlse_ctx* ctx, double* ms);
 
+/* Noise ensembles on mixed grids (engine-only: pyofss runs one System per
+ * realisation; this replaces that loop for the stepping path). Groups are up
+ * to four sets of realisations, each on the grid (4096, 8192 or 16384) and
+ * operators of one context; every realisation runs Stepper.adaptive_stepper
+ * (stepper.py:247-376) with its own step sizes, all in one launch. After the
+ * propagation the spectral coherence of each group of M >= 2 realisations,
+ * g12 = | |sum a_i|^2 - sum |a_i|^2 | / ((M-1) sum |a_i|^2) with a_i the
+ * pyofss spectrum fft(A_i) (field.py:109-121), is formed on the device.
+ * The contexts must outlive the ensemble. */
+typedef struct gnlse_ensemble gnlse_ensemble;
+int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroups, gnlse_ctx* const* ctxs,
+                          const int64_t* batch, gnlse_ensemble** out);
+/* threads = 512 or 1024 selects the launch shape; 0 times each candidate on
+ * trial_attempts replayed steps of h0 per realisation and keeps the fastest. */
+int gnlse_ensemble_tune(gnlse_ensemble* e, int threads, int trial_attempts, int* chosen);
+int gnlse_ensemble_threads(gnlse_ensemble* e, int* threads);
+/* device pointers of a group's inputs (batch x N complex128), outputs and g12 */
+int gnlse_ensemble_device_ptrs(gnlse_ensemble* e, int group, double** d_in, double** d_out, double** d_g12);
+int gnlse_ensemble_set_input(gnlse_ensemble* e, int group, const double* A_in);
+/* One run (inputs are not modified, so runs repeat). use_graph = 1 replays a
+ * hipGraph of the launch sequence, captured at the first such call. */
+int gnlse_ensemble_run(gnlse_ensemble* e, int use_graph);
+/* host copies of a group's outputs; any pointer may be NULL */
+int gnlse_ensemble_output(gnlse_ensemble* e, int group, double* A_out, double* g12, gnlse_instance_result* results);
+int gnlse_ensemble_last_ms(gnlse_ensemble* e, double* ms);
+int gnlse_ensemble_destroy(gnlse_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

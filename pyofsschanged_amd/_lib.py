@@ -100,6 +100,15 @@ SIGNATURES = {
     "gnlse_memcpy_dtoh": [_P, _P, _P, _I64],
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
+    "gnlse_ensemble_tune": [_P, _INT, _INT, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_ensemble_threads": [_P, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_ensemble_device_ptrs": [_P, _INT, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)],
+    "gnlse_ensemble_set_input": [_P, _INT, _P],
+    "gnlse_ensemble_run": [_P, _INT],
+    "gnlse_ensemble_output": [_P, _INT, _P, _P, _P],
+    "gnlse_ensemble_last_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gnlse_ensemble_destroy": [_P],
 }
 
 _lib = None
@@ -326,6 +335,90 @@ class Context:
     def last_kernel_ms(self):
         v = ctypes.c_double()
         self._call("gnlse_last_kernel_ms", ctypes.byref(v))
+        return v.value
+
+
+def _results(res):
+    return [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted, "n_traces": r.n_traces,
+             "status": r.status} for r in res]
+
+
+class Ensemble:
+    """Realisations on up to four grids (4096/8192/16384) advanced in one
+    launch, each with its own adaptive step sizes, plus the spectral
+    coherence |g12| of each group on the device.
+
+    groups: list of (context, batch); every context holds its grid's
+    operators and must outlive the ensemble."""
+
+    def __init__(self, method, params, groups):
+        self.lib = load_library()
+        self.groups = [(ctx, int(b)) for ctx, b in groups]
+        self.contexts = [ctx for ctx, _ in self.groups]  # keep them alive
+        handles = (ctypes.c_void_p * len(self.groups))(*[ctx._h for ctx, _ in self.groups])
+        batch = (ctypes.c_int64 * len(self.groups))(*[b for _, b in self.groups])
+        h = ctypes.c_void_p()
+        self.params = params
+        _check(self.lib, self.lib.gnlse_ensemble_create(int(method), ctypes.byref(params), len(self.groups),
+                                                        ctypes.cast(handles, ctypes.c_void_p),
+                                                        ctypes.cast(batch, ctypes.c_void_p), ctypes.byref(h)),
+               "gnlse_ensemble_create")
+        self._h = h
+        self.lock = threading.RLock()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gnlse_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        with self.lock:
+            _check(self.lib, getattr(self.lib, name)(self._h, *args), name)
+
+    def tune(self, threads=0, trial_attempts=3):
+        """threads 512 or 1024 fixes the launch shape; 0 times both on a short
+        trial and keeps the faster. Returns the chosen thread count."""
+        c = ctypes.c_int()
+        self._call("gnlse_ensemble_tune", int(threads), int(trial_attempts), ctypes.byref(c))
+        return c.value
+
+    @property
+    def threads(self):
+        c = ctypes.c_int()
+        self._call("gnlse_ensemble_threads", ctypes.byref(c))
+        return c.value
+
+    def device_ptrs(self, group):
+        a, b, g = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._call("gnlse_ensemble_device_ptrs", int(group), ctypes.byref(a), ctypes.byref(b), ctypes.byref(g))
+        return a.value, b.value, g.value
+
+    def set_input(self, group, A):
+        ctx, b = self.groups[group]
+        a = _c128(A, ctx.n).reshape(b, ctx.n)
+        self._call("gnlse_ensemble_set_input", int(group), _ptr(a))
+
+    def run(self, graph=False):
+        self._call("gnlse_ensemble_run", 1 if graph else 0)
+        return self.last_ms()
+
+    def output(self, group):
+        ctx, b = self.groups[group]
+        A = np.empty((b, ctx.n), dtype=np.complex128)
+        g12 = np.empty(ctx.n, dtype=np.float64)
+        res = (InstanceResult * b)()
+        self._call("gnlse_ensemble_output", int(group), _ptr(A), _ptr(g12), ctypes.cast(res, ctypes.c_void_p))
+        return {"A": A, "g12": g12, "results": _results(res)}
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        self._call("gnlse_ensemble_last_ms", ctypes.byref(v))
         return v.value
 
 

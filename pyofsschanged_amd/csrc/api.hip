@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
+#include "ensemble.hpp"
 
 namespace gn {
 const LaunchTable* table_256();
@@ -88,6 +88,14 @@ static int upload(gnlse_ctx* c, DevBuf& b, const void* src, size_t bytes) {
   RET_IF(ensure(b, bytes));
   HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+static int upload_raw(int device, hipStream_t s, DevBuf& b, const void* src, size_t bytes) {
+  HIPCHK(hipSetDevice(device));
+  RET_IF(ensure(b, bytes));
+  HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
   return GNLSE_OK;
 }
 
@@ -548,3 +556,316 @@ int gnlse_last_kernel_ms(gnlse_ctx* c, double* ms) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles: realisations on up to four grids in one launch (ensemble.hpp).
+// ---------------------------------------------------------------------------
+struct gnlse_ensemble {
+  int device = 0, method = 0, kind = NL_KERR, ngroups = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  gnlse_adaptive_params params{};
+  // caller's group order; slot[i] = position of caller group i in launch order
+  int64_t n[kEnsMaxGroups] = {}, batch[kEnsMaxGroups] = {};
+  int slot[kEnsMaxGroups] = {};
+  size_t field_off[kEnsMaxGroups] = {}, g12_off[kEnsMaxGroups] = {}, res_off[kEnsMaxGroups] = {};
+  int64_t blocks = 0;
+  std::vector<EnsembleGroup> groups;  // launch order (largest grid first)
+  DevBuf in, out, ws, spec, g12, res, dgroups, rp_h, rp_a;
+  const EnsembleTable* tab = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static const EnsembleTable* ensemble_for_threads(int t) {
+  if (t == 512) return ensemble_table_512();
+  if (t == 1024) return ensemble_table_1024();
+  return nullptr;
+}
+
+static void drop_graph(gnlse_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+// The launch sequence of one ensemble run: the propagation kernel, then one
+// coherence reduction per group of two or more realisations.
+static hipError_t ensemble_enqueue(gnlse_ensemble* e, const EnsembleGroup* dgroups) {
+  hipError_t r = e->tab->run(e->stream, e->method, e->kind, dgroups, e->ngroups, e->blocks);
+  if (r != hipSuccess) return r;
+  for (int i = 0; i < e->ngroups; ++i) {
+    const EnsembleGroup& g = e->groups[i];
+    if (g.count < 2) continue;
+    const unsigned nb = (unsigned)((g.n + 255) / 256);
+    hipLaunchKernelGGL((k_coherence<256>), dim3(nb), dim3(256), 0, e->stream, (const double2*)g.spec, g.count, g.n,
+                       (double*)((double*)e->g12.p + e->g12_off[i]));
+    r = hipGetLastError();
+    if (r != hipSuccess) return r;
+  }
+  return hipSuccess;
+}
+
+static int ensemble_time(gnlse_ensemble* e, const EnsembleGroup* dgroups, bool graph) {
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  if (graph) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  else {
+    hipError_t r = ensemble_enqueue(e, dgroups);
+    if (r != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
+  }
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  HIPCHK(hipEventSynchronize(e->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  e->last_ms = ms;
+  return GNLSE_OK;
+}
+
+extern "C" {
+
+int gnlse_ensemble_destroy(gnlse_ensemble* e) {
+  if (!e) return GNLSE_OK;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graph(e);
+  DevBuf* bufs[] = {&e->in, &e->out, &e->ws, &e->spec, &e->g12, &e->res, &e->dgroups, &e->rp_h, &e->rp_a};
+  for (DevBuf* b : bufs) release(*b);
+  if (e->ev0) (void)hipEventDestroy(e->ev0);
+  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroups, gnlse_ctx* const* ctxs,
+                          const int64_t* batch, gnlse_ensemble** out) {
+  if (!out) return fail(GNLSE_ERR_ARG, "null out");
+  *out = nullptr;
+  if (ngroups < 1 || ngroups > kEnsMaxGroups || !ctxs || !batch)
+    return fail(GNLSE_ERR_ARG, "an ensemble has 1 to 4 groups");
+  AdaptiveParams prm;
+  RET_IF(adaptive_params(method, p, prm));
+  for (int i = 0; i < ngroups; ++i) {
+    RET_IF(check_ready(ctxs[i], true, true));
+    const int64_t n = ctxs[i]->n;
+    if (n != 4096 && n != 8192 && n != 16384)
+      return fail(GNLSE_ERR_UNSUPPORTED, "ensemble grids are 4096, 8192 or 16384 samples, got " + std::to_string(n));
+    if (ctxs[i]->device != ctxs[0]->device) return fail(GNLSE_ERR_ARG, "ensemble groups must share one device");
+    if (batch[i] < 1 || batch[i] > (1 << 24)) return fail(GNLSE_ERR_ARG, "group batch out of range");
+  }
+  gnlse_ensemble* e = new gnlse_ensemble();
+  e->device = ctxs[0]->device;
+  e->method = method;
+  e->ngroups = ngroups;
+  e->params = *p;
+  if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+    gnlse_ensemble_destroy(e);
+    return fail(GNLSE_ERR_HIP, "stream/event creation failed");
+  }
+  // launch order: larger grids first (stable for equal sizes)
+  std::vector<int> order(ngroups);
+  for (int i = 0; i < ngroups; ++i) order[i] = i;
+  for (int a = 1; a < ngroups; ++a)
+    for (int b = a; b > 0 && ctxs[order[b]]->n > ctxs[order[b - 1]]->n; --b) std::swap(order[b], order[b - 1]);
+  size_t fields = 0, g12s = 0, ress = 0;
+  bool all_kerr = true;
+  for (int s = 0; s < ngroups; ++s) {
+    const int i = order[s];
+    e->slot[i] = s;
+    e->n[i] = ctxs[i]->n;
+    e->batch[i] = batch[i];
+    e->field_off[s] = fields;
+    e->g12_off[s] = g12s;
+    e->res_off[s] = ress;
+    fields += (size_t)batch[i] * ctxs[i]->n;
+    g12s += (size_t)ctxs[i]->n;
+    ress += (size_t)batch[i];
+    all_kerr = all_kerr && ctxs[i]->ops.nl_kind == NL_KERR;
+  }
+  e->kind = all_kerr ? NL_KERR : NL_GENERIC;
+  int rc = GNLSE_OK;
+  auto alloc = [&](DevBuf& b, size_t bytes) {
+    if (rc == GNLSE_OK) rc = ensure(b, bytes);
+  };
+  alloc(e->in, fields * sizeof(double2));
+  alloc(e->out, fields * sizeof(double2));
+  alloc(e->spec, fields * sizeof(double2));
+  alloc(e->ws, fields * NSLOT * sizeof(double2));
+  alloc(e->g12, g12s * sizeof(double));
+  alloc(e->res, ress * sizeof(gnlse_instance_result));
+  alloc(e->dgroups, kEnsMaxGroups * sizeof(EnsembleGroup));
+  if (rc != GNLSE_OK) {
+    gnlse_ensemble_destroy(e);
+    return rc;
+  }
+  if (hipMemsetAsync(e->in.p, 0, e->in.bytes, e->stream) != hipSuccess ||
+      hipMemsetAsync(e->g12.p, 0, e->g12.bytes, e->stream) != hipSuccess) {
+    gnlse_ensemble_destroy(e);
+    return fail(GNLSE_ERR_HIP, "ensemble buffer initialisation failed");
+  }
+  e->groups.resize(ngroups);
+  int64_t first = 0;
+  for (int s = 0; s < ngroups; ++s) {
+    const int i = order[s];
+    gnlse_ctx* c = ctxs[i];
+    refresh_ops(c);
+    EnsembleGroup& g = e->groups[s];
+    std::memset(&g, 0, sizeof(g));
+    g.ops = c->ops;
+    g.prm = prm;
+    g.prm.res = (gnlse_instance_result*)e->res.p + e->res_off[s];
+    g.in = (const double2*)e->in.p + e->field_off[s];
+    g.out = (double2*)e->out.p + e->field_off[s];
+    g.spec = (double2*)e->spec.p + e->field_off[s];
+    g.ws = (double2*)e->ws.p + e->field_off[s] * NSLOT;
+    g.n = c->n;
+    g.first = first;
+    g.count = batch[i];
+    first += batch[i];
+  }
+  e->blocks = first;
+  if (hipMemcpyAsync(e->dgroups.p, e->groups.data(), ngroups * sizeof(EnsembleGroup), hipMemcpyHostToDevice,
+                     e->stream) != hipSuccess ||
+      hipStreamSynchronize(e->stream) != hipSuccess) {
+    gnlse_ensemble_destroy(e);
+    return fail(GNLSE_ERR_HIP, "ensemble table upload failed");
+  }
+  e->tab = ensemble_table_1024();
+  *out = e;
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_tune(gnlse_ensemble* e, int threads, int trial_attempts, int* chosen) {
+  if (!e) return fail(GNLSE_ERR_ARG, "null ensemble");
+  HIPCHK(hipSetDevice(e->device));
+  if (threads != 0) {
+    const EnsembleTable* t = ensemble_for_threads(threads);
+    if (!t) return fail(GNLSE_ERR_ARG, "threads per realisation must be 0 (tune), 512 or 1024");
+    if (t != e->tab) drop_graph(e);
+    e->tab = t;
+    if (chosen) *chosen = t->t;
+    return GNLSE_OK;
+  }
+  if (trial_attempts < 1 || trial_attempts > 64) return fail(GNLSE_ERR_ARG, "trial_attempts must be 1..64");
+  // Trial: every realisation replays trial_attempts accepted steps of h0,
+  // a fixed amount of work per candidate; outputs are overwritten by the
+  // next run.
+  std::vector<double> rh(trial_attempts, e->params.h0);
+  std::vector<unsigned char> ra(trial_attempts, 1);
+  RET_IF(upload_raw(e->device, e->stream, e->rp_h, rh.data(), rh.size() * sizeof(double)));
+  RET_IF(upload_raw(e->device, e->stream, e->rp_a, ra.data(), ra.size()));
+  std::vector<EnsembleGroup> trial = e->groups;
+  for (EnsembleGroup& g : trial) {
+    g.prm.replay_h = (const double*)e->rp_h.p;
+    g.prm.replay_accept = (const unsigned char*)e->rp_a.p;
+    g.prm.replay_len = trial_attempts;
+  }
+  DevBuf tg;
+  RET_IF(ensure(tg, kEnsMaxGroups * sizeof(EnsembleGroup)));
+  int rc = GNLSE_OK;
+  if (hipMemcpyAsync(tg.p, trial.data(), e->ngroups * sizeof(EnsembleGroup), hipMemcpyHostToDevice, e->stream) !=
+      hipSuccess)
+    rc = fail(GNLSE_ERR_HIP, "trial table upload failed");
+  const EnsembleTable* cands[] = {ensemble_table_512(), ensemble_table_1024()};
+  const EnsembleTable* best = nullptr;
+  double best_ms = 0.0;
+  const EnsembleTable* keep = e->tab;
+  for (const EnsembleTable* t : cands) {
+    if (rc != GNLSE_OK) break;
+    e->tab = t;
+    double ms = 0.0;
+    for (int rep = 0; rep < 2 && rc == GNLSE_OK; ++rep) {  // the first run loads the code object
+      rc = ensemble_time(e, (const EnsembleGroup*)tg.p, false);
+      ms = e->last_ms;
+    }
+    if (rc == GNLSE_OK && (!best || ms < best_ms)) {
+      best = t;
+      best_ms = ms;
+    }
+  }
+  release(tg);
+  if (rc != GNLSE_OK) {
+    e->tab = keep;
+    return rc;
+  }
+  if (best != keep) drop_graph(e);
+  e->tab = best;
+  e->last_ms = best_ms;
+  if (chosen) *chosen = best->t;
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_threads(gnlse_ensemble* e, int* threads) {
+  if (!e || !threads) return fail(GNLSE_ERR_ARG, "null argument");
+  *threads = e->tab->t;
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_device_ptrs(gnlse_ensemble* e, int group, double** d_in, double** d_out, double** d_g12) {
+  if (!e || group < 0 || group >= e->ngroups) return fail(GNLSE_ERR_ARG, "bad ensemble or group");
+  const int s = e->slot[group];
+  if (d_in) *d_in = (double*)((double2*)e->in.p + e->field_off[s]);
+  if (d_out) *d_out = (double*)((double2*)e->out.p + e->field_off[s]);
+  if (d_g12) *d_g12 = (double*)e->g12.p + e->g12_off[s];
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_set_input(gnlse_ensemble* e, int group, const double* A_in) {
+  if (!e || !A_in || group < 0 || group >= e->ngroups) return fail(GNLSE_ERR_ARG, "bad ensemble, group or input");
+  HIPCHK(hipSetDevice(e->device));
+  const int s = e->slot[group];
+  HIPCHK(hipMemcpyAsync((double2*)e->in.p + e->field_off[s], A_in, (size_t)e->batch[group] * e->n[group] * sizeof(double2),
+                        hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_run(gnlse_ensemble* e, int use_graph) {
+  if (!e) return fail(GNLSE_ERR_ARG, "null ensemble");
+  HIPCHK(hipSetDevice(e->device));
+  if (use_graph && !e->exec) {
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t r = ensemble_enqueue(e, (const EnsembleGroup*)e->dgroups.p);
+    hipGraph_t g = nullptr;
+    hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+    if (r != hipSuccess || r2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(GNLSE_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(r != hipSuccess ? r : r2));
+    }
+    e->graph = g;
+    if (hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0) != hipSuccess) {
+      drop_graph(e);
+      return fail(GNLSE_ERR_HIP, "ensemble graph instantiation failed");
+    }
+  }
+  return ensemble_time(e, (const EnsembleGroup*)e->dgroups.p, use_graph != 0);
+}
+
+int gnlse_ensemble_output(gnlse_ensemble* e, int group, double* A_out, double* g12, gnlse_instance_result* results) {
+  if (!e || group < 0 || group >= e->ngroups) return fail(GNLSE_ERR_ARG, "bad ensemble or group");
+  HIPCHK(hipSetDevice(e->device));
+  const int s = e->slot[group];
+  if (A_out)
+    HIPCHK(hipMemcpyAsync(A_out, (double2*)e->out.p + e->field_off[s],
+                          (size_t)e->batch[group] * e->n[group] * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+  if (g12)
+    HIPCHK(hipMemcpyAsync(g12, (double*)e->g12.p + e->g12_off[s], (size_t)e->n[group] * sizeof(double),
+                          hipMemcpyDeviceToHost, e->stream));
+  if (results)
+    HIPCHK(hipMemcpyAsync(results, (gnlse_instance_result*)e->res.p + e->res_off[s],
+                          (size_t)e->batch[group] * sizeof(gnlse_instance_result), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_last_ms(gnlse_ensemble* e, double* ms) {
+  if (!e || !ms) return fail(GNLSE_ERR_ARG, "null argument");
+  *ms = e->last_ms;
+  return GNLSE_OK;
+}
+
+}  // extern "C"
+

@@ -1,0 +1,4 @@
+// Ensemble kernels (mixed 4096/8192/16384 grids, one launch) at 512 threads
+// per realisation.
+#include "ensemble.hpp"
+GN_DEFINE_ENSEMBLE(512)

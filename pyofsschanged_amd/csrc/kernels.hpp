@@ -179,12 +179,16 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 
 // stepper.py:247-376: step doubling, local extrapolation and the controller,
 // all per instance on the device.
+// One adaptive propagation (stepper.py:247-376) by the calling workgroup:
+// instance b of a batch whose inputs, outputs and workspaces start at in,
+// out and ws_all. With spec != nullptr the pyofss spectrum fft(A_out) is
+// also written to spec + b*N (input of the ensemble coherence).
 template <int N, int T, int METHOD, int K>
-__global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
-                                                double2* __restrict__ out, double2* ws_all) {
-  GN_SHARED(N, T, K);
+__device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
+                                              size_t b, const double2* __restrict__ in, double2* __restrict__ out,
+                                              double2* ws_all, double2* __restrict__ spec) {
+  Dev<N, T, K> d(lds, red, ops);
   constexpr int P = N / T;
-  const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
   double2* sA = ws + S_A * N;
   double2* sC = ws + S_C * N;
@@ -304,6 +308,10 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
   if (!finished && status == GNLSE_STATUS_OK) status = GNLSE_STATUS_MAX_STEPS;
   d.load(x, sA);
   d.store(out + b * N, x);
+  if (spec) {
+    d.fft(x);
+    d.store(spec + b * N, x);
+  }
   if (threadIdx.x == 0) {
     gnlse_instance_result r;
     r.z = z;
@@ -315,6 +323,14 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
     r.reserved = 0;
     prm.res[b] = r;
   }
+}
+
+template <int N, int T, int METHOD, int K>
+__global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
+                                                double2* __restrict__ out, double2* ws_all) {
+  __shared__ double lds[FFT<N, T>::LDS_DOUBLES];
+  __shared__ double red[T / 64 + 2];
+  adaptive_body<N, T, METHOD, K>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening

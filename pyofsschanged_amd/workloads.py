@@ -9,6 +9,11 @@
                 examples/dudley_sc/fig_3.py:35-36), self-steepening and
                 Blow-Wood Raman, 8192 samples over 12.5 ps, adaptive RK4IP with
                 local error 1e-6.
+  noise_ensemble  realisations of the soliton fibre (adaptive RK4IP, local
+                error 1e-6) on several grids over the same 100 ps window
+                (default 128 x 16384, 256 x 8192, 128 x 4096 samples), each
+                input with its own one-photon-per-mode noise; one launch,
+                |g12| per grid on the device.
 
 Input noise: one photon per mode with a uniform random spectral phase,
 |A~_k| = sqrt(h nu_k / (N dt)) (h = 6.626e-10 W ps^2, the constant of
@@ -22,6 +27,7 @@ import numpy as np
 
 from .domain import Domain
 from .field import fftshift, ifft
+from ._lib import Ensemble
 from .modules.fibre import Fibre
 from .modules.sech import Sech
 
@@ -56,3 +62,36 @@ def supercontinuum(samples=8192, length=1.5e-4, local_error=1e-6, traces=1):
     fibre = Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
                   method="ark4ip", local_error=local_error, traces=traces)
     return domain, pulse, fibre
+
+
+NOISE_ENSEMBLE_GROUPS = ((16384, 128), (8192, 256), (4096, 128))
+
+
+def noise_ensemble(groups=NOISE_ENSEMBLE_GROUPS, length=math.pi / 40, local_error=1e-6, base_seed=1000,
+                   method="ark4ip", inputs=None):
+    """Build the noise ensemble: one Fibre (and device context) per grid, the
+    inputs (sech + noise; realisation r of the g-th group uses seed
+    base_seed + sum of earlier group sizes + r) uploaded, and the Ensemble
+    handle. ``inputs`` may give the input arrays per group instead.
+    Returns (ensemble, setups) with setups[g] = dict(N, domain, fibre,
+    seeds, inputs)."""
+    setups = []
+    seed = base_seed
+    for g, (N, M) in enumerate(groups):
+        domain = Domain(1, N, 100.0, C_NM_PS / 1550.0)
+        fibre = Fibre(length=length, beta=[0.0, 0.0, -20.0], gamma=1.3, method=method, local_error=local_error)
+        fibre.prepare(domain)
+        seeds = list(range(seed, seed + M))
+        seed += M
+        if inputs is None:
+            A0 = Sech(peak_power=4 * 20.0 / 1.3, width=1.0).generate(domain.t)
+            A = np.stack([A0 + oppm_noise(domain, s) for s in seeds])
+        else:
+            A = np.asarray(inputs[g], dtype=np.complex128).reshape(M, N)
+        setups.append(dict(N=N, domain=domain, fibre=fibre, seeds=seeds, inputs=A))
+    stepper = setups[0]["fibre"].stepper
+    ens = Ensemble(stepper.solver.method_id, stepper.adaptive_params(),
+                   [(s["fibre"]._device_context(), len(s["seeds"])) for s in setups])
+    for g, s in enumerate(setups):
+        ens.set_input(g, s["inputs"])
+    return ens, setups
