@@ -202,6 +202,50 @@ int gnlse_ensemble_output(gnlse_ensemble* e, int group, double* A_out, double* g
 int gnlse_ensemble_last_ms(gnlse_ensemble* e, double* ms);
 int gnlse_ensemble_destroy(gnlse_ensemble* e);
 
+/* Spectral neural operator surrogate of a fibre segment (engine-only; no
+ * pyofss counterpart). A 1-D Fourier neural operator (lift to `width`
+ * channels, `layers` Fourier layers keeping `modes` modes, GELU, a projection
+ * through `hidden` channels to 2 outputs added to the first two inputs)
+ * trained and evaluated with bf16 matrix-core kernels and FP32 accumulation.
+ * Inputs x are B x n x in_channels FP32, outputs y B x n x 2 FP32, both in
+ * device memory. Parameters are one FP32 vector (gnlse_sno_param_count). */
+typedef struct gnlse_sno gnlse_sno;
+int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, int in_channels, int hidden,
+                     int64_t max_batch, gnlse_sno** out);
+int gnlse_sno_param_count(gnlse_sno* s, int64_t* count);
+int gnlse_sno_set_params(gnlse_sno* s, const float* params);
+int gnlse_sno_get_params(gnlse_sno* s, float* params);
+int gnlse_sno_forward(gnlse_sno* s, int64_t batch, const float* d_x, float* d_y);
+/* Forward, relative L2 loss mean_b |y_b - t_b|^2 / |t_b|^2 against targets
+ * d_t, and the gradient of every parameter (kept on the device; copied to
+ * `grad` when not NULL). rel (host, batch entries, may be NULL) receives
+ * |y_b - t_b| / |t_b|. batch must be a multiple of 32. */
+int gnlse_sno_loss_grad(gnlse_sno* s, int64_t batch, const float* d_x, const float* d_t, double* loss, double* rel,
+                        float* grad);
+/* One Adam step on the last gradient. */
+int gnlse_sno_adam(gnlse_sno* s, float lr, float beta1, float beta2, float eps);
+int gnlse_sno_last_ms(gnlse_sno* s, double* ms);
+int gnlse_sno_destroy(gnlse_sno* s);
+/* Field encoding on the device with the context c of the same grid: d_out
+ * (B x n x channels FP32) = the pyofss spectrum fft(A) (field.py:109-121) in
+ * natural frequency order, times d_phase (complex128, natural order, may be
+ * NULL), divided by scale; channel 2 (channels = 3) is the frequency
+ * coordinate in [-1, 1). decode inverts it for predictions d_y (B x n x 2)
+ * back to fields d_A (complex128, time domain). */
+int gnlse_sno_encode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const double* d_A, double scale, const double* d_phase,
+                     int channels, float* d_out);
+int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y, double scale, const double* d_phase,
+                     double* d_A);
+/* Per-field relative change of energy between inputs d_x and predictions d_y
+ * (the judge's invariant gate for a lossless segment), to host `defect`. */
+int gnlse_sno_energy_defect(gnlse_sno* s, int64_t batch, const float* d_x, const float* d_y, double* defect);
+/* C (FP32) = op(A) op(B) on bf16 operands (device pointers), batched with
+ * element strides sA, sB, sC and split over K; at/bt select stored-transposed
+ * operands (not both). K / ksplit must be a multiple of 32. */
+int gnlse_gemm_bf16(int device, int at, int bt, int64_t M, int64_t N, int64_t K, const void* d_A, int64_t lda,
+                    const void* d_B, int64_t ldb, float* d_C, int64_t ldc, int64_t batch, int64_t sA, int64_t sB,
+                    int64_t sC, int ksplit);
+
 #ifdef __cplusplus
 }
 #endif

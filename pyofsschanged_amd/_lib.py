@@ -109,6 +109,19 @@ SIGNATURES = {
     "gnlse_ensemble_output": [_P, _INT, _P, _P, _P],
     "gnlse_ensemble_last_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_ensemble_destroy": [_P],
+    "gnlse_sno_create": [_INT, _I64, _INT, _INT, _INT, _INT, _INT, _I64, ctypes.POINTER(_P)],
+    "gnlse_sno_param_count": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "gnlse_sno_set_params": [_P, _P],
+    "gnlse_sno_get_params": [_P, _P],
+    "gnlse_sno_forward": [_P, _I64, _P, _P],
+    "gnlse_sno_loss_grad": [_P, _I64, _P, _P, ctypes.POINTER(ctypes.c_double), _P, _P],
+    "gnlse_sno_adam": [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float],
+    "gnlse_sno_last_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gnlse_sno_destroy": [_P],
+    "gnlse_sno_encode": [_P, _P, _I64, _P, _D, _P, _INT, _P],
+    "gnlse_sno_decode": [_P, _P, _I64, _P, _D, _P, _P],
+    "gnlse_sno_energy_defect": [_P, _I64, _P, _P, _P],
+    "gnlse_gemm_bf16": [_INT, _INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _I64, _INT],
 }
 
 _lib = None

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "ensemble.hpp"
+#include "ctx.hpp"
 
 namespace gn {
 const LaunchTable* table_256();
@@ -29,111 +29,7 @@ const LaunchTable* find_table(long long n) {
 
 using namespace gn;
 
-static thread_local std::string g_err;
-
-static int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(GNLSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define RET_IF(x)                  \
-  do {                             \
-    int r_ = (x);                  \
-    if (r_ != GNLSE_OK) return r_; \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct gnlse_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
-  int64_t n = 0;
-  const LaunchTable* tab = nullptr;
-  DevBuf lin, efix, nlmul, hr, omega, gprof, simpw, tw;
-  double efix_h = NAN;
-  bool have_lin = false, have_nl = false;
-  Ops ops{};
-  DevBuf ws, io_in, io_out, io_b, tr_A, tr_z, st_z, st_h, lg_h, lg_d, lg_a, rp_h, rp_a, res, ntr;
-};
-
-static int ensure(DevBuf& b, size_t bytes) {
-  if (bytes == 0) bytes = 16;
-  if (b.bytes >= bytes) return GNLSE_OK;
-  if (b.p) HIPCHK(hipFree(b.p));
-  b.p = nullptr;
-  b.bytes = 0;
-  HIPCHK(hipMalloc(&b.p, bytes));
-  b.bytes = bytes;
-  return GNLSE_OK;
-}
-static void release(DevBuf& b) {
-  if (b.p) (void)hipFree(b.p);
-  b.p = nullptr;
-  b.bytes = 0;
-}
-
-static int upload(gnlse_ctx* c, DevBuf& b, const void* src, size_t bytes) {
-  RET_IF(ensure(b, bytes));
-  HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return GNLSE_OK;
-}
-
-static int upload_raw(int device, hipStream_t s, DevBuf& b, const void* src, size_t bytes) {
-  HIPCHK(hipSetDevice(device));
-  RET_IF(ensure(b, bytes));
-  HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return GNLSE_OK;
-}
-
-// exp(+2 pi i m / N) in extended precision, rounded to double.
-static int build_twiddles(gnlse_ctx* c) {
-  const int64_t n = c->n;
-  std::vector<double> tw(2 * n);
-  const long double two_pi = 6.283185307179586476925286766559005768L;
-  for (int64_t m = 0; m < n; ++m) {
-    const long double a = two_pi * (long double)m / (long double)n;
-    tw[2 * m] = (double)cosl(a);
-    tw[2 * m + 1] = (double)sinl(a);
-  }
-  return upload(c, c->tw, tw.data(), tw.size() * sizeof(double));
-}
-
-static void refresh_ops(gnlse_ctx* c) {
-  c->ops.lin = (const double2*)c->lin.p;
-  c->ops.efix = (const double2*)c->efix.p;
-  c->ops.nlmul = (const double2*)c->nlmul.p;
-  c->ops.hr = (const double2*)c->hr.p;
-  c->ops.omega = (const double*)c->omega.p;
-  c->ops.tw = (const double2*)c->tw.p;
-  c->ops.gprof = (const double*)c->gprof.p;
-  c->ops.simpw = (const double*)c->simpw.p;
-}
-
-static int check_ready(gnlse_ctx* c, bool need_lin, bool need_nl) {
-  if (!c) return fail(GNLSE_ERR_ARG, "null context");
-  if (c->n == 0 || !c->tab) return fail(GNLSE_ERR_STATE, "gnlse_set_grid has not been called");
-  if (need_lin && !c->have_lin) return fail(GNLSE_ERR_STATE, "gnlse_set_linear has not been called");
-  if (need_nl && !c->have_nl) return fail(GNLSE_ERR_STATE, "gnlse_set_nonlinear has not been called");
-  return GNLSE_OK;
-}
-
-static int ensure_ws(gnlse_ctx* c, int64_t batch) {
-  return ensure(c->ws, (size_t)batch * NSLOT * c->n * sizeof(double2));
-}
+thread_local std::string g_err;
 
 static int timed_begin(gnlse_ctx* c) {
   HIPCHK(hipEventRecord(c->ev0, c->stream));

@@ -1,0 +1,338 @@
+// Spectral neural operator (1-D Fourier neural operator) for fibre segments:
+// forward and backward kernels on bf16 matrix cores with FP32 accumulation.
+//
+// Network (x: B x N x CI input features, y: B x N x 2):
+//   H0 = x Wl + bl                                   (lift, CI -> C)
+//   for l = 1..L:
+//     V  = DFT_K(H)      truncated real DFT along the grid, modes k < K:
+//                        V[k] = sum_n H[n] exp(-2 pi i k n / N)
+//     Ws = V * R_k       complex C x C mixing per mode
+//     U  = IDFT_K(Ws)    u[n] = (1/N) sum_k c_k Re(Ws[k] exp(+2 pi i k n / N)),
+//                        c_0 = 1, c_k = 2 (the inverse real DFT of the
+//                        retained modes)
+//     Z  = H Wp + bp + U
+//     H  = gelu(Z) for l < L, Z for l = L
+//   Y1 = gelu(H Wq1 + bq1)                           (C -> CH)
+//   y  = x[..., 0:2] + Y1 Wq2 + bq2                  (residual output)
+// gelu(z) = z/2 (1 + erf(z/sqrt 2)).
+//
+// Every product with K >= 32 is a bf16 MFMA GEMM (mfma.hpp): the truncated
+// DFT and its inverse are GEMMs against fixed cos/sin matrices (Ft: 2K x N,
+// G: N x 2K), the per-mode complex mixing is a batch of real GEMMs against
+// [[Rr, Ri], [-Ri, Rr]], and the pointwise layers are GEMMs over all B*N
+// points. The epilogues below write each GEMM's result directly in the
+// layout the next GEMM reads, fused with bias, residual and activation.
+//
+// Layouts: activations [b][n][c] (channels contiguous); mixing operands
+// [k][b][2C] (real parts in the first C columns, imaginary in the last C);
+// DFT coefficients per field [b][2K][C] with row 2k = real, 2k+1 = imaginary.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mfma.hpp"
+
+namespace gn {
+
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+  return cdf + z * pdf;
+}
+
+// ---- GEMM epilogues -------------------------------------------------------
+// DFT forward (batch = field b, row = 2k + re/im, col = channel) -> [k][b][2C]
+struct EpiToModes {
+  uint16_t* out;
+  long long B, C;
+  __device__ void operator()(int b, long long row, long long col, float v) const {
+    out[((row >> 1) * B + b) * 2 * C + (row & 1) * C + col] = f2bf(v);
+  }
+};
+// mixing (batch = mode k, row = field b, col = j in [0, 2C)) -> [b][2K][C]
+struct EpiToFields {
+  uint16_t* out;
+  long long K2, C;  // K2 = 2K
+  __device__ void operator()(int k, long long b, long long col, float v) const {
+    const long long r = 2 * k + (col >= C ? 1 : 0);
+    out[(b * K2 + r) * C + (col % C)] = f2bf(v);
+  }
+};
+// plain FP32 store C[batch*sC + row*ldc + col] (optionally accumulate)
+struct EpiF32 {
+  float* out;
+  long long ldc, sC;
+  int accumulate;
+  __device__ void operator()(int z, long long row, long long col, float v) const {
+    float* p = out + z * sC + row * ldc + col;
+    *p = accumulate ? *p + v : v;
+  }
+};
+// inverse DFT (batch = field b, row = n, col = o) -> U[b][n][o] (FP32)
+struct EpiIdft {
+  float* U;
+  long long N, C;
+  __device__ void operator()(int b, long long n, long long o, float v) const { U[(b * N + n) * C + o] = v; }
+};
+// pointwise layer: Z = acc + bias + U; H = gelu(Z) (or Z on the last layer)
+struct EpiLayer {
+  const float* bias;
+  const float* U;
+  float* Z;  // may be null (inference)
+  uint16_t* H;
+  long long C;
+  int last;
+  __device__ void operator()(int, long long p, long long o, float v) const {
+    const long long i = p * C + o;
+    const float z = v + bias[o] + U[i];
+    if (Z) Z[i] = z;
+    H[i] = f2bf(last ? z : gelu_f(z));
+  }
+};
+// projection hidden layer: Zq = acc + bias; Y1 = gelu(Zq)
+struct EpiProj {
+  const float* bias;
+  float* Z;  // may be null
+  uint16_t* Y;
+  long long CH;
+  __device__ void operator()(int, long long p, long long h, float v) const {
+    const float z = v + bias[h];
+    if (Z) Z[p * CH + h] = z;
+    Y[p * CH + h] = f2bf(gelu_f(z));
+  }
+};
+
+// ---- elementwise / reductions ---------------------------------------------
+// H0 = x Wl + bl (CI inputs per point)
+__global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
+                                                 const float* __restrict__ bl, uint16_t* __restrict__ H, long long P,
+                                                 int CI, int C) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= P * C) return;
+  const long long p = i / C;
+  const int c = (int)(i % C);
+  float a = bl[c];
+  for (int j = 0; j < CI; ++j) a += x[p * CI + j] * Wl[j * C + c];
+  H[i] = f2bf(a);
+}
+
+// y[p][j] = x[p][j] + bq2[j] + sum_h Y1[p][h] Wq2[h][j], j = 0, 1
+__global__ void __launch_bounds__(256) k_sno_out(const uint16_t* __restrict__ Y1, const float* __restrict__ Wq2,
+                                                const float* __restrict__ bq2, const float* __restrict__ x,
+                                                float* __restrict__ y, long long P, int CH, int CI) {
+  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  float a0 = bq2[0] + x[p * CI], a1 = bq2[1] + x[p * CI + 1];
+  for (int h = 0; h < CH; ++h) {
+    const float v = bf2f(Y1[p * CH + h]);
+    a0 += v * Wq2[2 * h];
+    a1 += v * Wq2[2 * h + 1];
+  }
+  y[2 * p] = a0;
+  y[2 * p + 1] = a1;
+}
+
+// Relative L2 loss per field, its mean, and dL/dy:
+//   L = (1/B) sum_b |y_b - t_b|^2 / |t_b|^2,  dy_b = (2/B) (y_b - t_b) / |t_b|^2
+// One workgroup per field; rel[b] = |y_b - t_b| / |t_b|.
+__global__ void __launch_bounds__(256) k_sno_loss(const float* __restrict__ y, const float* __restrict__ t,
+                                                 float* __restrict__ dy, double* __restrict__ rel, long long M,
+                                                 long long B) {
+  __shared__ double s_num[256], s_den[256];
+  const long long b = blockIdx.x;
+  const float* yb = y + b * M;
+  const float* tb = t + b * M;
+  double num = 0.0, den = 0.0;
+  for (long long i = threadIdx.x; i < M; i += 256) {
+    const double d = (double)yb[i] - (double)tb[i];
+    num += d * d;
+    den += (double)tb[i] * tb[i];
+  }
+  s_num[threadIdx.x] = num;
+  s_den[threadIdx.x] = den;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_num[threadIdx.x] += s_num[threadIdx.x + o];
+      s_den[threadIdx.x] += s_den[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  const double N2 = s_num[0], D2 = s_den[0] > 0.0 ? s_den[0] : 1.0;
+  if (threadIdx.x == 0) rel[b] = sqrt(N2 / D2);
+  if (dy) {
+    const float sc = (float)(2.0 / ((double)B * D2));
+    for (long long i = threadIdx.x; i < M; i += 256) dy[b * M + i] = sc * (yb[i] - tb[i]);
+  }
+}
+
+// dZq[p][h] = (sum_j dy[p][j] Wq2[h][j]) * gelu'(Zq[p][h])
+__global__ void __launch_bounds__(256) k_sno_out_bwd(const float* __restrict__ dy, const float* __restrict__ Wq2,
+                                                    const float* __restrict__ Zq, uint16_t* __restrict__ dZq,
+                                                    float* __restrict__ dZq32, long long P, int CH) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= P * CH) return;
+  const long long p = i / CH;
+  const int h = (int)(i % CH);
+  const float g = (dy[2 * p] * Wq2[2 * h] + dy[2 * p + 1] * Wq2[2 * h + 1]) * gelu_grad(Zq[i]);
+  dZq[i] = f2bf(g);
+  dZq32[i] = g;
+}
+
+// dZ = dH * gelu'(Z) (or dH on the last layer), in bf16 and FP32
+__global__ void __launch_bounds__(256) k_sno_act_bwd(const float* __restrict__ dH, const float* __restrict__ Z,
+                                                    uint16_t* __restrict__ dZ, float* __restrict__ dZ32, long long n,
+                                                    int last) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float g = last ? dH[i] : dH[i] * gelu_grad(Z[i]);
+  dZ[i] = f2bf(g);
+  dZ32[i] = g;
+}
+
+// Partial sums over point chunks of a[p][i] * b[p][j] (a = 1 when null, or
+// bf16 when a16 is given): part[blk][i*NB + j]. NA*NB <= 512.
+__global__ void __launch_bounds__(256) k_sno_outer_partial(const float* __restrict__ a, const uint16_t* __restrict__ a16,
+                                                          const float* __restrict__ b, float* __restrict__ part,
+                                                          long long P, int NA, int NB, long long chunk) {
+  __shared__ float acc[512];
+  const int n = NA * NB;
+  for (int e = threadIdx.x; e < n; e += 256) acc[e] = 0.f;
+  __syncthreads();
+  const long long p0 = (long long)blockIdx.x * chunk, p1 = p0 + chunk < P ? p0 + chunk : P;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int i = e / NB, j = e % NB;
+    float s = 0.f;
+    for (long long p = p0; p < p1; ++p) {
+      const float av = a ? a[p * NA + i] : (a16 ? bf2f(a16[p * NA + i]) : 1.0f);
+      s += av * b[p * NB + j];
+    }
+    acc[e] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n; e += 256) part[(long long)blockIdx.x * n + e] = acc[e];
+}
+
+// out[i] = sum_s part[s][i] (fixed order)
+__global__ void __launch_bounds__(256) k_sno_sum_parts(const float* __restrict__ part, float* __restrict__ out,
+                                                      long long S, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (long long k = 0; k < S; ++k) s += part[k * n + i];
+  out[i] = s;
+}
+
+// [[Rr, Ri], [-Ri, Rr]] per mode, bf16
+__global__ void __launch_bounds__(256) k_sno_build_blk(const float* __restrict__ Rr, const float* __restrict__ Ri,
+                                                      uint16_t* __restrict__ blk, int K, int C) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long per = (long long)C * C;
+  if (i >= (long long)K * per) return;
+  const long long k = i / per;
+  const int r = (int)((i % per) / C), o = (int)(i % C);
+  const float re = Rr[i], im = Ri[i];
+  uint16_t* b = blk + k * 4 * per;
+  const int L = 2 * C;
+  b[r * L + o] = f2bf(re);
+  b[r * L + C + o] = f2bf(im);
+  b[(C + r) * L + o] = f2bf(-im);
+  b[(C + r) * L + C + o] = f2bf(re);
+}
+
+// dRr = dBlk[r][o] + dBlk[C+r][C+o]; dRi = dBlk[r][C+o] - dBlk[C+r][o]
+__global__ void __launch_bounds__(256) k_sno_fold_blk(const float* __restrict__ dblk, float* __restrict__ dRr,
+                                                     float* __restrict__ dRi, int K, int C) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long per = (long long)C * C;
+  if (i >= (long long)K * per) return;
+  const long long k = i / per;
+  const int r = (int)((i % per) / C), o = (int)(i % C);
+  const float* d = dblk + k * 4 * per;
+  const int L = 2 * C;
+  dRr[i] = d[r * L + o] + d[(C + r) * L + C + o];
+  dRi[i] = d[r * L + C + o] - d[(C + r) * L + o];
+}
+
+__global__ void __launch_bounds__(256) k_sno_to_bf16(const float* __restrict__ s, uint16_t* __restrict__ d, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) d[i] = f2bf(s[i]);
+}
+
+// Adam with bias correction (c1 = 1 - b1^t, c2 = 1 - b2^t precomputed)
+__global__ void __launch_bounds__(256) k_sno_adam(float* __restrict__ w, const float* __restrict__ g,
+                                                 float* __restrict__ m, float* __restrict__ v, long long n, float lr,
+                                                 float b1, float b2, float eps, float c1, float c2) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Encoding of fields for the network. A~ is the pyofss spectrum fft(A) in
+// FFT order; the network sees it in natural frequency order (index n holds
+// FFT bin (n + N/2) mod N), multiplied by an optional per-frequency phase
+// (natural order) and divided by `scale`; channel 2, when present, is the
+// frequency coordinate (n - N/2) / (N/2).
+__global__ void __launch_bounds__(256) k_sno_encode(const double2* __restrict__ spec, const double2* __restrict__ phase,
+                                                   double scale, float* __restrict__ out, long long B, long long N,
+                                                   int channels) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * N) return;
+  const long long b = i / N, n = i % N;
+  double2 a = spec[b * N + ((n + N / 2) % N)];
+  if (phase) {
+    const double2 p = phase[n];
+    a = make_double2(a.x * p.x - a.y * p.y, a.x * p.y + a.y * p.x);
+  }
+  out[i * channels] = (float)(a.x / scale);
+  out[i * channels + 1] = (float)(a.y / scale);
+  if (channels > 2) out[i * channels + 2] = (float)((double)(n - N / 2) / (double)(N / 2));
+}
+
+// Inverse of the encoding for predictions y (B x N x 2): spectrum in FFT
+// order = scale * (y0 + i y1) * phase[n] at bin (n + N/2) mod N.
+__global__ void __launch_bounds__(256) k_sno_decode(const float* __restrict__ y, const double2* __restrict__ phase,
+                                                   double scale, double2* __restrict__ spec, long long B, long long N) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * N) return;
+  const long long b = i / N, n = i % N;
+  double2 a = make_double2(scale * (double)y[2 * i], scale * (double)y[2 * i + 1]);
+  if (phase) {
+    const double2 p = phase[n];
+    a = make_double2(a.x * p.x - a.y * p.y, a.x * p.y + a.y * p.x);
+  }
+  spec[b * N + ((n + N / 2) % N)] = a;
+}
+
+// Per-field relative energy change |sum |y|^2 - sum |x_{0,1}|^2| / sum |x_{0,1}|^2
+// (the segment is lossless, so a faithful prediction keeps its energy).
+__global__ void __launch_bounds__(256) k_sno_energy_defect(const float* __restrict__ x, const float* __restrict__ y,
+                                                          double* __restrict__ out, long long N, int CI) {
+  __shared__ double s_in[256], s_out[256];
+  const long long b = blockIdx.x;
+  double ein = 0.0, eout = 0.0;
+  for (long long n = threadIdx.x; n < N; n += 256) {
+    const long long p = b * N + n;
+    const double x0 = x[p * CI], x1 = x[p * CI + 1], y0 = y[2 * p], y1 = y[2 * p + 1];
+    ein += x0 * x0 + x1 * x1;
+    eout += y0 * y0 + y1 * y1;
+  }
+  s_in[threadIdx.x] = ein;
+  s_out[threadIdx.x] = eout;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_in[threadIdx.x] += s_in[threadIdx.x + o];
+      s_out[threadIdx.x] += s_out[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = s_in[0] > 0.0 ? fabs(s_out[0] - s_in[0]) / s_in[0] : 0.0;
+}
+
+}  // namespace gn

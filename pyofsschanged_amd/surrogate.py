@@ -1,0 +1,305 @@
+"""Learned surrogate of a fibre segment: a spectral neural operator trained
+on propagations the engine itself computes, with the FP64 adaptive RK4IP
+solver as its judge and fallback.
+
+Problem (fixed for one trained network): the soliton fibre's physics
+(beta2 = -20 ps^2/km, gamma = 1.3 /W/km, Kerr) over a segment of length L
+(default 0.5 m) on 4096 samples over a 12.5 ps window at 1550 nm. The input
+family: sech pulses with peak power log-uniform in [1, 10] kW, FWHM uniform
+in [30, 150] fs, chirp uniform in [-1, 1], each with seeded one-photon-per-
+mode noise. (At the 100 ps / 4096-sample grid of the 1000-step soliton, dt =
+24.4 fs would not resolve a 30 fs pulse; over the whole soliton period such
+pulses disperse beyond any window of 4096 samples that resolves them, hence
+the 12.5 ps window and the 0.5 m segment.)
+
+Representation: the network sees the pyofss spectrum of the input,
+A~_in(nu) / s in natural frequency order with the frequency as a third
+channel, and predicts exp(-D L) A~_out(nu) / s (the output in the
+interaction picture of the linear operator D, whose action is known
+exactly), as a residual on the input. s is a fixed amplitude scale of the
+training set.
+
+Judging: every prediction passes the lossless-segment energy gate (relative
+energy change of the prediction <= gate); predictions failing it are
+re-solved. A fixed fraction (every audit_every-th field) is re-solved as
+well and its relative spectral error |S_pred - S_ref| / |S_ref| (S =
+|A~|^2) measured; audited predictions above the tolerance are replaced by
+the solver's field and counted.
+"""
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .domain import Domain
+from .field import fftshift
+from .modules.fibre import Fibre
+
+C_NM_PS = Domain.vacuum_light_speed
+PLANCK_W_PS2 = 6.626e-10
+
+
+def _off(p, nbytes):
+    return ctypes.c_void_p(p.value + int(nbytes))
+
+
+class DeviceArray:
+    """A device allocation owned through an engine context."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctx.malloc(max(self.nbytes, 16))
+
+    def at(self, index):
+        """Pointer to row `index` of the leading axis."""
+        row = self.nbytes // self.shape[0]
+        return _off(self.ptr, row * index)
+
+    def upload(self, host, index=0):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        self.ctx.htod(self.at(index), host)
+
+    def download(self, index=0, count=None):
+        count = self.shape[0] - index if count is None else count
+        out = np.empty((count,) + self.shape[1:], dtype=self.dtype)
+        self.ctx.dtoh(out, self.at(index))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            self.ctx.free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class SpectralOperator:
+    """The network on the device (gnlse_sno_*). Parameters are one FP32 vector
+    in the layout of `layout()`."""
+
+    def __init__(self, n=4096, width=64, modes=128, layers=4, in_channels=3, hidden=128, max_batch=256, device=0):
+        self.lib = _lib.load_library()
+        if _lib.device_count() < 1:
+            raise _lib.EngineUnavailableError("no HIP device is visible to the GNLSE engine")
+        self.n, self.width, self.modes, self.layers = n, width, modes, layers
+        self.in_channels, self.hidden, self.max_batch = in_channels, hidden, max_batch
+        h = ctypes.c_void_p()
+        _lib._check(self.lib, self.lib.gnlse_sno_create(device, n, width, modes, layers, in_channels, hidden, max_batch,
+                                                        ctypes.byref(h)), "gnlse_sno_create")
+        self._h = h
+        c = ctypes.c_int64()
+        self._call("gnlse_sno_param_count", ctypes.byref(c))
+        self.size = c.value
+
+    def _call(self, name, *args):
+        _lib._check(self.lib, getattr(self.lib, name)(self._h, *args), name)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.gnlse_sno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        """name -> (offset, shape), the order of the C parameter vector."""
+        C, K, CH = self.width, self.modes, self.hidden
+        out, o = {}, 0
+        for name, shape in ([("Wl", (self.in_channels, C)), ("bl", (C,))]
+                            + [x for l in range(self.layers) for x in
+                               ((f"Rr{l}", (K, C, C)), (f"Ri{l}", (K, C, C)), (f"Wp{l}", (C, C)), (f"bp{l}", (C,)))]
+                            + [("Wq1", (C, CH)), ("bq1", (CH,)), ("Wq2", (CH, 2)), ("bq2", (2,))]):
+            out[name] = (o, shape)
+            o += int(np.prod(shape))
+        assert o == self.size
+        return out
+
+    def init_params(self, seed=0):
+        """Fan-in uniform initialisation; spectral weights U(0, 1)/(C*C) as in
+        the published Fourier neural operator; the output layer starts at
+        zero, so the untrained network is the identity map of the residual."""
+        rng = np.random.default_rng(seed)
+        w = np.zeros(self.size, dtype=np.float32)
+        for name, (o, shape) in self.layout().items():
+            n = int(np.prod(shape))
+            if name.startswith("R"):
+                v = rng.uniform(0.0, 1.0, n) / (self.width * self.width)
+            elif name.startswith("b") or name == "Wq2":
+                v = np.zeros(n)
+            else:
+                v = rng.uniform(-1.0, 1.0, n) / math.sqrt(shape[0])
+            w[o:o + n] = v
+        self.set_params(w)
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.size
+        self._call("gnlse_sno_set_params", w.ctypes.data_as(ctypes.c_void_p))
+
+    def get_params(self):
+        w = np.empty(self.size, dtype=np.float32)
+        self._call("gnlse_sno_get_params", w.ctypes.data_as(ctypes.c_void_p))
+        return w
+
+    def forward(self, batch, d_x, d_y):
+        self._call("gnlse_sno_forward", int(batch), d_x, d_y)
+
+    def loss_grad(self, batch, d_x, d_t, want_grad=False):
+        loss = ctypes.c_double()
+        rel = np.empty(batch, dtype=np.float64)
+        g = np.empty(self.size, dtype=np.float32) if want_grad else None
+        self._call("gnlse_sno_loss_grad", int(batch), d_x, d_t, ctypes.byref(loss), rel.ctypes.data_as(ctypes.c_void_p),
+                   None if g is None else g.ctypes.data_as(ctypes.c_void_p))
+        return loss.value, rel, g
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._call("gnlse_sno_adam", ctypes.c_float(lr), ctypes.c_float(beta1), ctypes.c_float(beta2),
+                   ctypes.c_float(eps))
+
+    def energy_defect(self, batch, d_x, d_y):
+        out = np.empty(batch, dtype=np.float64)
+        self._call("gnlse_sno_energy_defect", int(batch), d_x, d_y, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        self._call("gnlse_sno_last_ms", ctypes.byref(v))
+        return v.value
+
+
+class SegmentProblem:
+    """The fibre segment the surrogate stands in for, its input family, and
+    the device-side encoding of fields."""
+
+    def __init__(self, samples=4096, window=12.5, wavelength=1550.0, length=5e-4, beta=(0.0, 0.0, -20.0),
+                 gamma=1.3, local_error=1e-6):
+        self.domain = Domain(1, samples, window, C_NM_PS / wavelength)
+        self.length = length
+        self.fibre = Fibre(length=length, beta=list(beta), gamma=gamma, method="ark4ip", local_error=local_error)
+        self.ctx = self.fibre.prepare(self.domain)
+        self.n = samples
+        D = np.asarray(self.fibre.linearity.factor, dtype=np.complex128)  # FFT order
+        nat = np.fft.fftshift(D)  # natural frequency order
+        self.phase_in = DeviceArray(self.ctx, (samples,), np.complex128)  # exp(-D L): output -> interaction picture
+        self.phase_out = DeviceArray(self.ctx, (samples,), np.complex128)
+        self.phase_in.upload(np.exp(-nat * length))
+        self.phase_out.upload(np.exp(nat * length))
+        self.params = self.fibre.stepper.adaptive_params()
+        self.method = self.fibre.stepper.solver.method_id
+
+    def sample(self, count, seed):
+        """Input fields (host, complex128) and their parameters."""
+        rng = np.random.default_rng(seed)
+        P0 = np.exp(rng.uniform(np.log(1e3), np.log(1e4), count))
+        fwhm = rng.uniform(0.030, 0.150, count)
+        chirp = rng.uniform(-1.0, 1.0, count)
+        t = self.domain.t
+        T0 = fwhm / (2.0 * np.log(1.0 + np.sqrt(2.0)))  # sech FWHM -> T0 (sech.py:111-113)
+        tau = t[None, :] / T0[:, None]
+        A = np.sqrt(P0)[:, None] / np.cosh(tau) * np.exp(-0.5j * chirp[:, None] * tau ** 2)
+        # one photon per mode, phases from Philox(key = seed * 2**20 + i)
+        nu = fftshift(self.domain.nu)
+        amp = np.sqrt(PLANCK_W_PS2 * nu / (self.n * self.domain.dt))
+        spec = np.empty((count, self.n), dtype=np.complex128)
+        for i in range(count):
+            g = np.random.Generator(np.random.Philox(key=seed * (1 << 20) + i))
+            spec[i] = amp * np.exp(1j * g.uniform(0.0, 2.0 * np.pi, self.n))
+        A = A + self.ctx.fft(spec, -1)
+        return A, dict(P0=P0, fwhm=fwhm, chirp=chirp)
+
+    def solve(self, d_in, d_out, count):
+        """FP64 adaptive RK4IP of `count` fields (device pointers)."""
+        res = self.ctx.propagate_adaptive_device(self.method, count, self.params, d_in, d_out)
+        bad = [r["status"] for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError(f"solver failed for {len(bad)} fields (status {bad[0]})")
+        return res
+
+    def encode(self, net, count, d_A, scale, d_out, channels=3, interaction=False):
+        _lib._check(net.lib, net.lib.gnlse_sno_encode(net._h, self.ctx._h, int(count), d_A, float(scale),
+                                                      self.phase_in.ptr if interaction else None, int(channels), d_out),
+                    "gnlse_sno_encode")
+
+    def decode(self, net, count, d_y, scale, d_A):
+        _lib._check(net.lib, net.lib.gnlse_sno_decode(net._h, self.ctx._h, int(count), d_y, float(scale),
+                                                      self.phase_out.ptr, d_A), "gnlse_sno_decode")
+
+
+def spectral_error(A_pred, A_ref):
+    """|S_pred - S_ref| / |S_ref| per field, S = |fft(A)|^2 (any FFT
+    normalisation cancels)."""
+    Sp = np.abs(np.fft.fft(A_pred, axis=-1)) ** 2
+    Sr = np.abs(np.fft.fft(A_ref, axis=-1)) ** 2
+    return np.linalg.norm(Sp - Sr, axis=-1) / np.linalg.norm(Sr, axis=-1)
+
+
+class Dataset:
+    """Solver-generated pairs, encoded on the device: x (count x n x 3) and
+    targets t (count x n x 2), FP32, plus the raw fields for judging."""
+
+    def __init__(self, problem, net, count, seed, scale=None, chunk=2048, keep_fields=False):
+        ctx = problem.ctx
+        n = problem.n
+        self.count = count
+        self.x = DeviceArray(ctx, (count, n, 3), np.float32)
+        self.t = DeviceArray(ctx, (count, n, 2), np.float32)
+        A_in = DeviceArray(ctx, (min(chunk, count), n), np.complex128)
+        A_out = DeviceArray(ctx, (min(chunk, count), n), np.complex128)
+        self.fields_in, self.fields_out = [], []
+        self.solve_s = 0.0
+        self.attempts = []
+        A_host, self.params = problem.sample(count, seed)
+        if scale is None:
+            scale = float(np.max(np.abs(np.fft.ifft(A_host[: min(count, 512)], axis=-1))))
+        self.scale = scale
+        for s0 in range(0, count, chunk):
+            m = min(chunk, count - s0)
+            A_in.upload(A_host[s0:s0 + m])
+            t0 = time.perf_counter()
+            res = problem.solve(A_in.ptr, A_out.ptr, m)
+            self.solve_s += time.perf_counter() - t0
+            self.attempts += [r["n_attempts"] for r in res]
+            problem.encode(net, m, A_in.ptr, scale, self.x.at(s0), channels=3)
+            problem.encode(net, m, A_out.ptr, scale, self.t.at(s0), channels=2, interaction=True)
+            if keep_fields:
+                self.fields_in.append(A_host[s0:s0 + m])
+                self.fields_out.append(A_out.download(0, m))
+        if keep_fields:
+            self.fields_in = np.concatenate(self.fields_in)
+            self.fields_out = np.concatenate(self.fields_out)
+        A_in.free()
+        A_out.free()
+
+
+def train(net, data, steps, batch=32, lr=1e-3, lr_end=1e-5, seed=0, log_every=0):
+    """Adam with a cosine learning-rate schedule over `steps` minibatches of
+    consecutive fields (the sampled dataset is already in random order).
+    Returns the loss history."""
+    rng = np.random.default_rng(seed)
+    nblk = data.count // batch
+    order = rng.permutation(nblk)
+    hist = []
+    for s in range(steps):
+        if s % nblk == 0 and s > 0:
+            order = rng.permutation(nblk)
+        b0 = int(order[s % nblk]) * batch
+        loss, _, _ = net.loss_grad(batch, data.x.at(b0), data.t.at(b0))
+        rate = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * s / max(1, steps - 1)))
+        net.adam(rate)
+        hist.append(loss)
+        if log_every and (s % log_every == 0 or s == steps - 1):
+            print(f"  step {s}: loss {loss:.3e} lr {rate:.2e}", flush=True)
+    return np.array(hist)

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     txt = open(os.path.join(ROOT, "include", "gnlse.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(gnlse_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(gnlse_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_function():
