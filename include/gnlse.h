@@ -226,16 +226,19 @@ int gnlse_sno_loss_grad(gnlse_sno* s, int64_t batch, const float* d_x, const flo
 int gnlse_sno_adam(gnlse_sno* s, float lr, float beta1, float beta2, float eps);
 int gnlse_sno_last_ms(gnlse_sno* s, double* ms);
 int gnlse_sno_destroy(gnlse_sno* s);
-/* Field encoding on the device with the context c of the same grid: d_out
- * (B x n x channels FP32) = the pyofss spectrum fft(A) (field.py:109-121) in
- * natural frequency order, times d_phase (complex128, natural order, may be
- * NULL), divided by scale; channel 2 (channels = 3) is the frequency
- * coordinate in [-1, 1). decode inverts it for predictions d_y (B x n x 2)
- * back to fields d_A (complex128, time domain). */
+/* Field encoding on the device with the context c of the same grid.
+ * Spectral mode (time_domain = 0): d_out (B x n x channels FP32) = the pyofss
+ * spectrum fft(A) (field.py:109-121) in natural frequency order, times
+ * d_phase (complex128, natural order, may be NULL), divided by scale.
+ * Time mode (time_domain = 1): d_out = ifft(d_phase * fft(A)) / scale on the
+ * time grid (d_phase in FFT order; NULL means A itself). Channel 2
+ * (channels = 3) is the grid coordinate in [-1, 1). decode inverts either
+ * mode for predictions d_y (B x n x 2) back to fields d_A (complex128, time
+ * domain). */
 int gnlse_sno_encode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const double* d_A, double scale, const double* d_phase,
-                     int channels, float* d_out);
+                     int channels, int time_domain, float* d_out);
 int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y, double scale, const double* d_phase,
-                     double* d_A);
+                     int time_domain, double* d_A);
 /* Per-field relative change of energy between inputs d_x and predictions d_y
  * (the judge's invariant gate for a lossless segment), to host `defect`. */
 int gnlse_sno_energy_defect(gnlse_sno* s, int64_t batch, const float* d_x, const float* d_y, double* defect);

@@ -118,8 +118,8 @@ SIGNATURES = {
     "gnlse_sno_adam": [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float],
     "gnlse_sno_last_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_sno_destroy": [_P],
-    "gnlse_sno_encode": [_P, _P, _I64, _P, _D, _P, _INT, _P],
-    "gnlse_sno_decode": [_P, _P, _I64, _P, _D, _P, _P],
+    "gnlse_sno_encode": [_P, _P, _I64, _P, _D, _P, _INT, _INT, _P],
+    "gnlse_sno_decode": [_P, _P, _I64, _P, _D, _P, _INT, _P],
     "gnlse_sno_energy_defect": [_P, _I64, _P, _P, _P],
     "gnlse_gemm_bf16": [_INT, _INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _I64, _INT],
 }

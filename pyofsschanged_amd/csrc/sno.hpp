@@ -103,33 +103,60 @@ struct EpiProj {
 };
 
 // ---- elementwise / reductions ---------------------------------------------
-// H0 = x Wl + bl (CI inputs per point)
+// H0 = x Wl + bl (CI inputs per point); 8 consecutive channels per thread,
+// stored as one 16-byte write.
 __global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
                                                  const float* __restrict__ bl, uint16_t* __restrict__ H, long long P,
                                                  int CI, int C) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= P * C) return;
-  const long long p = i / C;
-  const int c = (int)(i % C);
-  float a = bl[c];
-  for (int j = 0; j < CI; ++j) a += x[p * CI + j] * Wl[j * C + c];
-  H[i] = f2bf(a);
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const long long p = i / groups;
+  const int c0 = (int)(i % groups) * 8;
+  float xi[8];
+  for (int j = 0; j < CI; ++j) xi[j] = x[p * CI + j];
+  uint16_t o[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float a = bl[c0 + c];
+    for (int j = 0; j < CI; ++j) a += xi[j] * Wl[j * C + c0 + c];
+    o[c] = f2bf(a);
+  }
+  *reinterpret_cast<uint4*>(H + p * C + c0) = *reinterpret_cast<const uint4*>(o);
 }
 
-// y[p][j] = x[p][j] + bq2[j] + sum_h Y1[p][h] Wq2[h][j], j = 0, 1
+// y[p][j] = x[p][j] + bq2[j] + sum_h Y1[p][h] Wq2[h][j], j = 0, 1. Sixteen
+// lanes share a point (each reads 8 consecutive hidden values as one 16-byte
+// load, so a wavefront reads 4 whole rows of Y1); partial sums are combined
+// across the 16 lanes. CH must be a multiple of 128 / 16 * 8 = 128 or CH/8
+// lanes are idle; CH <= 128 is handled by lane masking.
 __global__ void __launch_bounds__(256) k_sno_out(const uint16_t* __restrict__ Y1, const float* __restrict__ Wq2,
                                                 const float* __restrict__ bq2, const float* __restrict__ x,
                                                 float* __restrict__ y, long long P, int CH, int CI) {
-  const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (p >= P) return;
-  float a0 = bq2[0] + x[p * CI], a1 = bq2[1] + x[p * CI + 1];
-  for (int h = 0; h < CH; ++h) {
-    const float v = bf2f(Y1[p * CH + h]);
-    a0 += v * Wq2[2 * h];
-    a1 += v * Wq2[2 * h + 1];
+  const long long p = ((long long)blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int q = threadIdx.x & 15;
+  float a0 = 0.f, a1 = 0.f;
+  if (p < P) {
+    for (int h0 = q * 8; h0 < CH; h0 += 128) {
+      const uint4 v = *reinterpret_cast<const uint4*>(Y1 + p * CH + h0);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float f = bf2f(e[j]);
+        a0 += f * Wq2[2 * (h0 + j)];
+        a1 += f * Wq2[2 * (h0 + j) + 1];
+      }
+    }
   }
-  y[2 * p] = a0;
-  y[2 * p + 1] = a1;
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, 16);
+    a1 += __shfl_xor(a1, o, 16);
+  }
+  if (p < P && q == 0) {
+    y[2 * p] = a0 + bq2[0] + x[p * CI];
+    y[2 * p + 1] = a1 + bq2[1] + x[p * CI + 1];
+  }
 }
 
 // Relative L2 loss per field, its mean, and dL/dy:
@@ -191,26 +218,32 @@ __global__ void __launch_bounds__(256) k_sno_act_bwd(const float* __restrict__ d
 }
 
 // Partial sums over point chunks of a[p][i] * b[p][j] (a = 1 when null, or
-// bf16 when a16 is given): part[blk][i*NB + j]. NA*NB <= 512.
+// bf16 when a16 is given): part[blk][i*NB + j], NA*NB <= 256. The 256
+// threads form 256/(NA*NB) point lanes; thread e of a lane owns element
+// (i, j) = (e / NB, e % NB), so consecutive threads read consecutive columns
+// of the same rows. Lane partials are combined in LDS in a fixed order.
 __global__ void __launch_bounds__(256) k_sno_outer_partial(const float* __restrict__ a, const uint16_t* __restrict__ a16,
                                                           const float* __restrict__ b, float* __restrict__ part,
                                                           long long P, int NA, int NB, long long chunk) {
-  __shared__ float acc[512];
+  __shared__ float acc[256];
   const int n = NA * NB;
-  for (int e = threadIdx.x; e < n; e += 256) acc[e] = 0.f;
-  __syncthreads();
+  const int lanes = 256 / n;
+  const int e = threadIdx.x % n, lane = threadIdx.x / n;
+  const int i = e / NB, j = e % NB;
   const long long p0 = (long long)blockIdx.x * chunk, p1 = p0 + chunk < P ? p0 + chunk : P;
-  for (int e = threadIdx.x; e < n; e += 256) {
-    const int i = e / NB, j = e % NB;
-    float s = 0.f;
-    for (long long p = p0; p < p1; ++p) {
+  float s = 0.f;
+  if (lane < lanes)
+    for (long long p = p0 + lane; p < p1; p += lanes) {
       const float av = a ? a[p * NA + i] : (a16 ? bf2f(a16[p * NA + i]) : 1.0f);
       s += av * b[p * NB + j];
     }
-    acc[e] = s;
-  }
+  acc[threadIdx.x] = (lane < lanes) ? s : 0.f;
   __syncthreads();
-  for (int e = threadIdx.x; e < n; e += 256) part[(long long)blockIdx.x * n + e] = acc[e];
+  if ((int)threadIdx.x < n) {
+    float t = 0.f;
+    for (int l = 0; l < lanes; ++l) t += acc[l * n + threadIdx.x];
+    part[(long long)blockIdx.x * n + threadIdx.x] = t;
+  }
 }
 
 // out[i] = sum_s part[s][i] (fixed order)
@@ -277,7 +310,8 @@ __global__ void __launch_bounds__(256) k_sno_adam(float* __restrict__ w, const f
 // FFT order; the network sees it in natural frequency order (index n holds
 // FFT bin (n + N/2) mod N), multiplied by an optional per-frequency phase
 // (natural order) and divided by `scale`; channel 2, when present, is the
-// frequency coordinate (n - N/2) / (N/2).
+// frequency coordinate (n - N/2) / (N/2) and channel 3, when present,
+// |a|^2 / scale^2.
 __global__ void __launch_bounds__(256) k_sno_encode(const double2* __restrict__ spec, const double2* __restrict__ phase,
                                                    double scale, float* __restrict__ out, long long B, long long N,
                                                    int channels) {
@@ -292,6 +326,7 @@ __global__ void __launch_bounds__(256) k_sno_encode(const double2* __restrict__ 
   out[i * channels] = (float)(a.x / scale);
   out[i * channels + 1] = (float)(a.y / scale);
   if (channels > 2) out[i * channels + 2] = (float)((double)(n - N / 2) / (double)(N / 2));
+  if (channels > 3) out[i * channels + 3] = (float)((a.x * a.x + a.y * a.y) / (scale * scale));
 }
 
 // Inverse of the encoding for predictions y (B x N x 2): spectrum in FFT
@@ -307,6 +342,35 @@ __global__ void __launch_bounds__(256) k_sno_decode(const float* __restrict__ y,
     a = make_double2(a.x * p.x - a.y * p.y, a.x * p.y + a.y * p.x);
   }
   spec[b * N + ((n + N / 2) % N)] = a;
+}
+
+// Time-domain encoding: out[b][n] = A[b][n] / scale (time grid in natural
+// order), channel 2 = (n - N/2) / (N/2), channel 3 = |A|^2 / scale^2;
+// decoding writes A = scale * y.
+__global__ void __launch_bounds__(256) k_sno_pack_time(const double2* __restrict__ A, double scale, float* __restrict__ out,
+                                                      long long B, long long N, int channels) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * N) return;
+  const long long n = i % N;
+  const double2 a = A[i];
+  out[i * channels] = (float)(a.x / scale);
+  out[i * channels + 1] = (float)(a.y / scale);
+  if (channels > 2) out[i * channels + 2] = (float)((double)(n - N / 2) / (double)(N / 2));
+  if (channels > 3) out[i * channels + 3] = (float)((a.x * a.x + a.y * a.y) / (scale * scale));
+}
+__global__ void __launch_bounds__(256) k_sno_unpack_time(const float* __restrict__ y, double scale,
+                                                        double2* __restrict__ A, long long total) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  A[i] = make_double2(scale * (double)y[2 * i], scale * (double)y[2 * i + 1]);
+}
+// spec[b][k] *= phase[k] (FFT order)
+__global__ void __launch_bounds__(256) k_sno_cmul(double2* __restrict__ spec, const double2* __restrict__ phase,
+                                                 long long B, long long N) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= B * N) return;
+  const double2 a = spec[i], p = phase[i % N];
+  spec[i] = make_double2(a.x * p.x - a.y * p.y, a.x * p.y + a.y * p.x);
 }
 
 // Per-field relative energy change |sum |y|^2 - sum |x_{0,1}|^2| / sum |x_{0,1}|^2

@@ -61,7 +61,7 @@ int sno_refresh_bf16(gnlse_sno* s) {
 // sum over points of a[p][i] * b[p][j] into out[i*NB + j]
 int sno_outer(gnlse_sno* s, const float* a, const uint16_t* a16, const float* b, long long P, int NA, int NB,
               float* out) {
-  const long long nblk = P < 256 ? P : 256;
+  const long long nblk = P < 1024 ? P : 1024;
   const long long chunk = (P + nblk - 1) / nblk;
   hipLaunchKernelGGL(k_sno_outer_partial, dim3((unsigned)nblk), dim3(256), 0, s->stream, a, a16, b, (float*)s->part.p,
                      P, NA, NB, chunk);
@@ -88,7 +88,7 @@ int sno_wgrad(gnlse_sno* s, const uint16_t* A, long long lda, const uint16_t* B,
 int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train) {
   const long long N = s->N, C = s->C, K = s->K, P = B * N;
   uint16_t* H = (uint16_t*)s->H.p;
-  hipLaunchKernelGGL(k_sno_lift, dim3(blocks_for(P * C)), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
+  hipLaunchKernelGGL(k_sno_lift, dim3(blocks_for(P * (C / 8))), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
                      W(s, s->lay.bl), H, P, (int)s->CI, (int)C);
   HIPCHK(hipGetLastError());
   for (long long l = 0; l < s->L; ++l) {
@@ -118,7 +118,7 @@ int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train)
                                              GemmShape{P, s->CH, C, C, s->CH, 0, 0, 1}, 1,
                                              EpiProj{W(s, s->lay.bq1), train ? (float*)s->Zq.p : nullptr,
                                                      (uint16_t*)s->Y1.p, s->CH})));
-  hipLaunchKernelGGL(k_sno_out, dim3(blocks_for(P)), dim3(256), 0, s->stream, (const uint16_t*)s->Y1.p,
+  hipLaunchKernelGGL(k_sno_out, dim3(blocks_for(P * 16)), dim3(256), 0, s->stream, (const uint16_t*)s->Y1.p,
                      W(s, s->lay.Wq2), W(s, s->lay.bq2), x, y, P, (int)s->CH, (int)s->CI);
   HIPCHK(hipGetLastError());
   return GNLSE_OK;
@@ -215,8 +215,8 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
   if (!out) return fail(GNLSE_ERR_ARG, "null out");
   *out = nullptr;
   if (n < 64 || n % 32 || width < 32 || width % 32 || modes < 16 || modes % 16 || 2 * modes > n || layers < 1 ||
-      layers > kSnoMaxLayers || in_channels < 2 || in_channels * width > 512 || hidden < 64 || hidden % 64 ||
-      2 * hidden > 512 || max_batch < 1)
+      layers > kSnoMaxLayers || in_channels < 2 || in_channels > 8 || in_channels * width > 256 || hidden < 64 || hidden % 64 ||
+      2 * hidden > 256 || max_batch < 1)
     return fail(GNLSE_ERR_ARG, "unsupported surrogate shape");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
@@ -391,37 +391,73 @@ int gnlse_sno_last_ms(gnlse_sno* s, double* ms) {
 }
 
 int gnlse_sno_encode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const double* d_A, double scale, const double* d_phase,
-                     int channels, float* d_out) {
-  if (!s || !d_A || !d_out || batch < 1 || !(scale > 0.0) || (channels != 2 && channels != 3))
+                     int channels, int time_domain, float* d_out) {
+  if (!s || !d_A || !d_out || batch < 1 || !(scale > 0.0) || channels < 2 || channels > 4)
     return fail(GNLSE_ERR_ARG, "invalid encode arguments");
   RET_IF(check_ready(c, false, false));
   if (c->n != s->N) return fail(GNLSE_ERR_ARG, "context grid differs from the surrogate grid");
   HIPCHK(hipSetDevice(s->device));
   RET_IF(ensure(c->io_b, (size_t)batch * s->N * sizeof(double2)));
   HIPCHK(hipStreamSynchronize(s->stream));
-  // pyofss fft on the context's stream, then the encoding on the same stream
-  hipError_t e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, (double2*)c->io_b.p);
+  const unsigned nb = blocks_for(batch * s->N);
+  double2* tmp = (double2*)c->io_b.p;
+  hipError_t e = hipSuccess;
+  if (!time_domain) {
+    // pyofss fft, then the natural-order encoding (phase in natural order)
+    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(k_sno_encode, dim3(nb), dim3(256), 0, c->stream, (const double2*)tmp,
+                         (const double2*)d_phase, scale, d_out, (long long)batch, s->N, channels);
+  } else if (d_phase) {
+    // ifft(phase * fft(A)) (phase in FFT order), then the time encoding
+    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_sno_cmul, dim3(nb), dim3(256), 0, c->stream, tmp, (const double2*)d_phase, (long long)batch,
+                         s->N);
+      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, tmp);
+    }
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(k_sno_pack_time, dim3(nb), dim3(256), 0, c->stream, (const double2*)tmp, scale, d_out,
+                         (long long)batch, s->N, channels);
+  } else {
+    hipLaunchKernelGGL(k_sno_pack_time, dim3(nb), dim3(256), 0, c->stream, (const double2*)d_A, scale, d_out,
+                       (long long)batch, s->N, channels);
+  }
   if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("fft: ") + hipGetErrorString(e));
-  hipLaunchKernelGGL(k_sno_encode, dim3(blocks_for(batch * s->N)), dim3(256), 0, c->stream,
-                     (const double2*)c->io_b.p, (const double2*)d_phase, scale, d_out, (long long)batch, s->N, channels);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }
 
 int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y, double scale, const double* d_phase,
-                     double* d_A) {
+                     int time_domain, double* d_A) {
   if (!s || !d_y || !d_A || batch < 1 || !(scale > 0.0)) return fail(GNLSE_ERR_ARG, "invalid decode arguments");
   RET_IF(check_ready(c, false, false));
   if (c->n != s->N) return fail(GNLSE_ERR_ARG, "context grid differs from the surrogate grid");
   HIPCHK(hipSetDevice(s->device));
   RET_IF(ensure(c->io_b, (size_t)batch * s->N * sizeof(double2)));
   HIPCHK(hipStreamSynchronize(s->stream));
-  hipLaunchKernelGGL(k_sno_decode, dim3(blocks_for(batch * s->N)), dim3(256), 0, c->stream, d_y,
-                     (const double2*)d_phase, scale, (double2*)c->io_b.p, (long long)batch, s->N);
+  const unsigned nb = blocks_for(batch * s->N);
+  double2* tmp = (double2*)c->io_b.p;
+  hipError_t e = hipSuccess;
+  if (!time_domain) {
+    hipLaunchKernelGGL(k_sno_decode, dim3(nb), dim3(256), 0, c->stream, d_y, (const double2*)d_phase, scale, tmp,
+                       (long long)batch, s->N);
+    e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A);
+  } else if (d_phase) {
+    hipLaunchKernelGGL(k_sno_unpack_time, dim3(nb), dim3(256), 0, c->stream, d_y, scale, tmp, (long long)batch * s->N);
+    e = c->tab->fft(c->stream, c->ops, +1, batch, tmp, tmp);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_sno_cmul, dim3(nb), dim3(256), 0, c->stream, tmp, (const double2*)d_phase, (long long)batch,
+                         s->N);
+      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A);
+    }
+  } else {
+    hipLaunchKernelGGL(k_sno_unpack_time, dim3(nb), dim3(256), 0, c->stream, d_y, scale, (double2*)d_A,
+                       (long long)batch * s->N);
+  }
+  if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("fft: ") + hipGetErrorString(e));
   HIPCHK(hipGetLastError());
-  hipError_t e = c->tab->fft(c->stream, c->ops, -1, batch, (const double2*)c->io_b.p, (double2*)d_A);
-  if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("ifft: ") + hipGetErrorString(e));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }
