@@ -203,7 +203,7 @@ int gnlse_ensemble_last_ms(gnlse_ensemble* e, double* ms);
 int gnlse_ensemble_destroy(gnlse_ensemble* e);
 
 /* Spectral neural operator surrogate of a fibre segment (engine-only; no
- * pyofss counterpart). A 1-D Fourier neural operator (lift to `width`
+ * pyofss counterpart). A 1-D Fourier neural operator (lift to `width` = 64
  * channels, `layers` Fourier layers keeping `modes` modes, GELU, a projection
  * through `hidden` channels to 2 outputs added to the first two inputs)
  * trained and evaluated with bf16 matrix-core kernels and FP32 accumulation.

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace gn {
 
@@ -50,144 +51,191 @@ struct GemmShape {
   int ksplit;
 };
 
+template <int BM, int BN>
+struct MmaTile {
+  static constexpr int BK = 32;
+  static constexpr int LD = BK + 8;  // padded LDS row (bf16)
+  static constexpr int WM = BM / 2, WN = BN / 2;  // per-wavefront tile
+  static constexpr int TM = WM / 16, TN = WN / 16;
+  static_assert(BM % 32 == 0 && BN % 32 == 0, "tile");
+};
+
+// Operand loaders. Each stages a ROWS x 32 slice of op(X) (rows = M or N
+// index, columns = k) into LDS as rows of 32 k-contiguous bf16 (row pitch
+// LD). A slice is fetched into registers (fetch) while the previous one is
+// multiplied, then written to LDS (stash).
+//
+// RowSrc: the source is k-contiguous, X[r * ld + k] (16-byte loads, 16-byte
+// LDS writes). Rows r >= rows are zero.
+template <int ROWS>
+struct RowSrc {
+  static constexpr int LOADS = ROWS * 32 / 8 / 256;
+  static_assert(LOADS >= 1, "tile too small for 256 threads");
+  const uint16_t* X;
+  long long ld, rows, r0;
+  uint4 reg[LOADS];
+  __device__ __forceinline__ void fetch(long long k0) {
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int row = i / 4, kc = (i % 4) * 8;
+      const long long r = r0 + row;
+      reg[q] = (r < rows) ? *reinterpret_cast<const uint4*>(X + r * ld + k0 + kc) : make_uint4(0, 0, 0, 0);
+    }
+  }
+  __device__ __forceinline__ void stash(uint16_t* lds) const {
+    constexpr int LD = 40;
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int row = i / 4, kc = (i % 4) * 8;
+      *reinterpret_cast<uint4*>(&lds[row * LD + kc]) = reg[q];
+    }
+  }
+};
+
+// ColSrc: the source is row-contiguous, X[k * ld + r] (16-byte loads of 8
+// consecutive rows of one k, transposed into LDS element by element).
+template <int ROWS>
+struct ColSrc {
+  static constexpr int LOADS = ROWS * 32 / 8 / 256;
+  static_assert(LOADS >= 1, "tile too small for 256 threads");
+  const uint16_t* X;
+  long long ld, rows, r0;
+  uint4 reg[LOADS];
+  __device__ __forceinline__ void fetch(long long k0) {
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int k = i / (ROWS / 8), rc = (i % (ROWS / 8)) * 8;
+      const long long r = r0 + rc;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (r + 8 <= rows) v = *reinterpret_cast<const uint4*>(X + (k0 + k) * ld + r);
+      else if (r < rows) {
+        uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < 8 && r + j < rows; ++j) t[j] = X[(k0 + k) * ld + r + j];
+        v = *reinterpret_cast<const uint4*>(t);
+      }
+      reg[q] = v;
+    }
+  }
+  __device__ __forceinline__ void stash(uint16_t* lds) const {
+    constexpr int LD = 40;
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int k = i / (ROWS / 8), rc = (i % (ROWS / 8)) * 8;
+      const uint16_t* t = reinterpret_cast<const uint16_t*>(&reg[q]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) lds[(rc + j) * LD + k] = t[j];
+    }
+  }
+};
+
+// Row2Src: two k-contiguous sources concatenated along k: k < K0 reads
+// X0[r * ld0 + k], k >= K0 reads X1[r * ld1 + k - K0] (K0 a multiple of 32,
+// so a slice never straddles the two).
+template <int ROWS>
+struct Row2Src {
+  static constexpr int LOADS = ROWS * 32 / 8 / 256;
+  static_assert(LOADS >= 1, "tile too small for 256 threads");
+  const uint16_t* X0;
+  const uint16_t* X1;
+  long long ld0, ld1, K0, rows, r0;
+  uint4 reg[LOADS];
+  __device__ __forceinline__ void fetch(long long k0) {
+    const bool first = k0 < K0;
+    const uint16_t* X = first ? X0 : X1;
+    const long long ld = first ? ld0 : ld1;
+    const long long kk = first ? k0 : k0 - K0;
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int row = i / 4, kc = (i % 4) * 8;
+      const long long r = r0 + row;
+      reg[q] = (r < rows) ? *reinterpret_cast<const uint4*>(X + r * ld + kk + kc) : make_uint4(0, 0, 0, 0);
+    }
+  }
+  __device__ __forceinline__ void stash(uint16_t* lds) const {
+    constexpr int LD = 40;
+#pragma unroll
+    for (int q = 0; q < LOADS; ++q) {
+      const int i = threadIdx.x + 256 * q;
+      const int row = i / 4, kc = (i % 4) * 8;
+      *reinterpret_cast<uint4*>(&lds[row * LD + kc]) = reg[q];
+    }
+  }
+};
+
+// The MFMA main loop over k in [k_begin, k_end) for one BM x BN tile:
+// 4 wavefronts (2 x 2), each accumulating (BM/2) x (BN/2) as 16 x 16 blocks
+// of v_mfma_f32_16x16x32_bf16. Accumulator block (i, j) of wavefront
+// (wm, wn) holds rows wm*BM/2 + 16 i + 4 (lane / 16) + v, column
+// wn*BN/2 + 16 j + lane % 16 in element v.
+template <int BM, int BN, class LA, class LB>
+__device__ __forceinline__ void mma_loop(LA& la, LB& lb, long long k_begin, long long k_end, uint16_t* As,
+                                         uint16_t* Bs, floatx4_t (&acc)[MmaTile<BM, BN>::TM][MmaTile<BM, BN>::TN]) {
+  using G = MmaTile<BM, BN>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+  for (int i = 0; i < G::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < G::TN; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+  const int lr = lane & 15, kq = (lane >> 4) * 8;
+  la.fetch(k_begin);
+  lb.fetch(k_begin);
+  for (long long k0 = k_begin; k0 < k_end; k0 += G::BK) {
+    __syncthreads();
+    la.stash(As);
+    lb.stash(Bs);
+    __syncthreads();
+    if (k0 + G::BK < k_end) {
+      la.fetch(k0 + G::BK);
+      lb.fetch(k0 + G::BK);
+    }
+    bf16x8_t a[G::TM], b[G::TN];
+#pragma unroll
+    for (int i = 0; i < G::TM; ++i)
+      a[i] = *reinterpret_cast<const bf16x8_t*>(&As[(wm * G::WM + i * 16 + lr) * G::LD + kq]);
+#pragma unroll
+    for (int j = 0; j < G::TN; ++j)
+      b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[(wn * G::WN + j * 16 + lr) * G::LD + kq]);
+#pragma unroll
+    for (int i = 0; i < G::TM; ++i)
+#pragma unroll
+      for (int j = 0; j < G::TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
 template <int BM, int BN, bool AT, bool BT, class Epi>
 __global__ void __launch_bounds__(256) k_gemm_bf16(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
                                                   GemmShape s, Epi epi) {
-  constexpr int BK = 32;
-  constexpr int LD = BK + 8;  // padded LDS row (bf16)
-  constexpr int WM = BM / 2, WN = BN / 2;  // per-wavefront tile
-  constexpr int TM = WM / 16, TN = WN / 16;
-  static_assert(BM % 32 == 0 && BN % 32 == 0, "tile");
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LD];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LD];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  using G = MmaTile<BM, BN>;
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * G::LD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * G::LD];
   const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
   const int z = blockIdx.z;
   const int batch = z / s.ksplit, split = z % s.ksplit;
   const long long Ks = s.K / s.ksplit, k_begin = (long long)split * Ks;
   const uint16_t* Ab = A + batch * s.sA;
   const uint16_t* Bb = B + batch * s.sB;
-
-  // Global -> register staging. Non-transposed A (and transposed B) rows are
-  // k-contiguous: 8 bf16 per 16-byte load. Transposed A (and non-transposed
-  // B) are m- (n-) contiguous: 8 consecutive rows of one k per load, written
-  // to LDS element by element.
-  constexpr int A_LOADS = BM * BK / 8 / 256;
-  constexpr int B_LOADS = BN * BK / 8 / 256;
-  static_assert(A_LOADS >= 1 && B_LOADS >= 1, "tile too small for 256 threads");
-  uint4 ra[A_LOADS], rb[B_LOADS];
-
-  auto fetch = [&](long long k0) {
+  using LA = typename std::conditional<AT, ColSrc<BM>, RowSrc<BM>>::type;
+  using LB = typename std::conditional<BT, RowSrc<BN>, ColSrc<BN>>::type;
+  LA la{Ab, s.lda, s.M, m0};
+  LB lb{Bb, s.ldb, s.N, n0};
+  floatx4_t acc[G::TM][G::TN];
+  mma_loop<BM, BN>(la, lb, k_begin, k_begin + Ks, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
-    for (int r = 0; r < A_LOADS; ++r) {
-      const int i = tid + 256 * r;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if constexpr (!AT) {
-        const int row = i / (BK / 8), kc = (i % (BK / 8)) * 8;
-        const long long m = m0 + row;
-        if (m < s.M) v = *reinterpret_cast<const uint4*>(Ab + m * s.lda + k0 + kc);
-      } else {
-        const int k = i / (BM / 8), mc = (i % (BM / 8)) * 8;
-        const long long m = m0 + mc;
-        if (m + 8 <= s.M) v = *reinterpret_cast<const uint4*>(Ab + (k0 + k) * s.lda + m);
-        else if (m < s.M) {
-          uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int j = 0; j < 8 && m + j < s.M; ++j) t[j] = Ab[(k0 + k) * s.lda + m + j];
-          v = *reinterpret_cast<const uint4*>(t);
-        }
-      }
-      ra[r] = v;
-    }
+  for (int i = 0; i < G::TM; ++i)
 #pragma unroll
-    for (int r = 0; r < B_LOADS; ++r) {
-      const int i = tid + 256 * r;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if constexpr (BT) {
-        const int row = i / (BK / 8), kc = (i % (BK / 8)) * 8;
-        const long long n = n0 + row;
-        if (n < s.N) v = *reinterpret_cast<const uint4*>(Bb + n * s.ldb + k0 + kc);
-      } else {
-        const int k = i / (BN / 8), nc = (i % (BN / 8)) * 8;
-        const long long n = n0 + nc;
-        if (n + 8 <= s.N) v = *reinterpret_cast<const uint4*>(Bb + (k0 + k) * s.ldb + n);
-        else if (n < s.N) {
-          uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int j = 0; j < 8 && n + j < s.N; ++j) t[j] = Bb[(k0 + k) * s.ldb + n + j];
-          v = *reinterpret_cast<const uint4*>(t);
-        }
-      }
-      rb[r] = v;
-    }
-  };
-  auto stash = [&]() {
-#pragma unroll
-    for (int r = 0; r < A_LOADS; ++r) {
-      const int i = tid + 256 * r;
-      if constexpr (!AT) {
-        const int row = i / (BK / 8), kc = (i % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(&As[row * LD + kc]) = ra[r];
-      } else {
-        const int k = i / (BM / 8), mc = (i % (BM / 8)) * 8;
-        const uint16_t* t = reinterpret_cast<const uint16_t*>(&ra[r]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) As[(mc + j) * LD + k] = t[j];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < B_LOADS; ++r) {
-      const int i = tid + 256 * r;
-      if constexpr (BT) {
-        const int row = i / (BK / 8), kc = (i % (BK / 8)) * 8;
-        *reinterpret_cast<uint4*>(&Bs[row * LD + kc]) = rb[r];
-      } else {
-        const int k = i / (BN / 8), nc = (i % (BN / 8)) * 8;
-        const uint16_t* t = reinterpret_cast<const uint16_t*>(&rb[r]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) Bs[(nc + j) * LD + k] = t[j];
-      }
-    }
-  };
-
-  floatx4_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-
-  // lane's operand rows/cols and k-chunk inside a 16 x 16 x 32 block
-  const int lr = lane & 15, kq = (lane >> 4) * 8;
-  fetch(k_begin);
-  for (long long k0 = k_begin; k0 < k_begin + Ks; k0 += BK) {
-    __syncthreads();
-    stash();
-    __syncthreads();
-    if (k0 + BK < k_begin + Ks) fetch(k0 + BK);
-    bf16x8_t a[TM], b[TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-      a[i] = *reinterpret_cast<const bf16x8_t*>(&As[(wm * WM + i * 16 + lr) * LD + kq]);
-#pragma unroll
-    for (int j = 0; j < TN; ++j)
-      b[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[(wn * WN + j * 16 + lr) * LD + kq]);
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-  }
-
-  // accumulator layout of a 16 x 16 block: column = lane % 16, rows
-  // 4 * (lane / 16) + v for v = 0..3
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const long long col = n0 + wn * WN + j * 16 + (lane & 15);
+    for (int j = 0; j < G::TN; ++j) {
+      const long long col = n0 + wn * G::WN + j * 16 + (lane & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const long long row = m0 + wm * WM + i * 16 + (lane >> 4) * 4 + v;
+        const long long row = m0 + wm * G::WM + i * 16 + (lane >> 4) * 4 + v;
         if (row < s.M && col < s.N) epi(z, row, col, acc[i][j][v]);
       }
     }

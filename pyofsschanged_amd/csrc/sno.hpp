@@ -58,6 +58,15 @@ struct EpiToFields {
     out[(b * K2 + r) * C + (col % C)] = f2bf(v);
   }
 };
+// mixing (batch = mode k, row = field b, col = j) -> WsT[b][j % C][2k + j / C]
+// (k-contiguous per output channel: the B operand of the fused layer GEMM)
+struct EpiToFieldsT {
+  uint16_t* out;
+  long long K2, C;
+  __device__ void operator()(int k, long long b, long long col, float v) const {
+    out[(b * C + (col % C)) * K2 + 2 * k + (col >= C ? 1 : 0)] = f2bf(v);
+  }
+};
 // plain FP32 store C[batch*sC + row*ldc + col] (optionally accumulate)
 struct EpiF32 {
   float* out;
@@ -102,17 +111,78 @@ struct EpiProj {
   }
 };
 
+// ---- fused Fourier layer ----------------------------------------------------
+// One layer's output for a 128-point x C-channel tile of field b (blockIdx.y):
+//   Z = [G | Hr_b] [WsT_b | WpT]^T + bias = IDFT_K(Ws) + H Wp + bias
+// as one GEMM with k = 2K + C (both segments k-contiguous), then
+// H' = gelu(Z) (Z on the last layer) written row-major and, through an LDS
+// staging tile, channel-first for the next layer's DFT. Z (FP32) is kept for
+// the backward pass when requested.
+template <int C>
+__global__ void __launch_bounds__(256) k_sno_layer(const uint16_t* __restrict__ G, const uint16_t* __restrict__ WsT,
+                                                  const uint16_t* __restrict__ Hr, const uint16_t* __restrict__ WpT,
+                                                  const float* __restrict__ bias, float* __restrict__ Z,
+                                                  uint16_t* __restrict__ Hr_next, uint16_t* __restrict__ Hc_next,
+                                                  long long N, long long K2, int last) {
+  using Tl = MmaTile<128, C>;
+  __shared__ __attribute__((aligned(16))) uint16_t As[128 * Tl::LD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[C * Tl::LD];
+  constexpr int TP = 128 + 8;  // staging row pitch (bf16), 16-byte multiple
+  __shared__ __attribute__((aligned(16))) uint16_t T[C * TP];
+  const long long b = blockIdx.y, n0 = (long long)blockIdx.x * 128;
+  Row2Src<128> la{G, Hr + b * N * C, K2, C, K2, N, n0};
+  Row2Src<C> lb{WsT + b * C * K2, WpT, K2, C, K2, C, 0};
+  floatx4_t acc[Tl::TM][Tl::TN];
+  mma_loop<128, C>(la, lb, 0, K2 + C, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+  for (int i = 0; i < Tl::TM; ++i)
+#pragma unroll
+    for (int j = 0; j < Tl::TN; ++j) {
+      const int o = wn * Tl::WN + j * 16 + (lane & 15);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * Tl::WM + i * 16 + (lane >> 4) * 4 + v;
+        const long long idx = (b * N + n0 + r) * C + o;
+        const float z = acc[i][j][v] + bias[o];
+        if (Z) Z[idx] = z;
+        const uint16_t h = f2bf(last ? z : gelu_f(z));
+        Hr_next[idx] = h;
+        T[o * TP + r] = h;
+      }
+    }
+  __syncthreads();
+  for (int q = threadIdx.x; q < C * 16; q += 256) {
+    const int o = q / 16, c8 = (q % 16) * 8;
+    *reinterpret_cast<uint4*>(Hc_next + (b * C + o) * N + n0 + c8) = *reinterpret_cast<const uint4*>(&T[o * TP + c8]);
+  }
+}
+
+// dst[c][r] = bf16(src[r][c]) for an R x Cc FP32 matrix
+__global__ void __launch_bounds__(256) k_sno_to_bf16_t(const float* __restrict__ src, uint16_t* __restrict__ dst,
+                                                      int R, int Cc) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long long)R * Cc) return;
+  const int r = (int)(i / Cc), c = (int)(i % Cc);
+  dst[(long long)c * R + r] = f2bf(src[i]);
+}
+
 // ---- elementwise / reductions ---------------------------------------------
-// H0 = x Wl + bl (CI inputs per point); 8 consecutive channels per thread,
-// stored as one 16-byte write.
+// H0 = x Wl + bl (CI inputs per point), written row-major (Hr[b][n][c], 8
+// channels per thread as one 16-byte store) and channel-first (Hc[b][c][n];
+// consecutive threads hold consecutive points of one channel group, so those
+// 2-byte stores are contiguous across the wavefront).
 __global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
-                                                 const float* __restrict__ bl, uint16_t* __restrict__ H, long long P,
-                                                 int CI, int C) {
+                                                 const float* __restrict__ bl, uint16_t* __restrict__ Hr,
+                                                 uint16_t* __restrict__ Hc, long long P, long long N, int CI, int C) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   const int groups = C / 8;
   if (i >= P * groups) return;
-  const long long p = i / groups;
-  const int c0 = (int)(i % groups) * 8;
+  const int g = (int)(i / P);
+  const long long p = i % P;
+  const long long b = p / N, n = p % N;
+  const int c0 = g * 8;
   float xi[8];
   for (int j = 0; j < CI; ++j) xi[j] = x[p * CI + j];
   uint16_t o[8];
@@ -121,8 +191,9 @@ __global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, c
     float a = bl[c0 + c];
     for (int j = 0; j < CI; ++j) a += xi[j] * Wl[j * C + c0 + c];
     o[c] = f2bf(a);
+    Hc[(b * C + c0 + c) * N + n] = o[c];
   }
-  *reinterpret_cast<uint4*>(H + p * C + c0) = *reinterpret_cast<const uint4*>(o);
+  *reinterpret_cast<uint4*>(Hr + p * C + c0) = *reinterpret_cast<const uint4*>(o);
 }
 
 // y[p][j] = x[p][j] + bq2[j] + sum_h Y1[p][h] Wq2[h][j], j = 0, 1. Sixteen
@@ -256,9 +327,10 @@ __global__ void __launch_bounds__(256) k_sno_sum_parts(const float* __restrict__
   out[i] = s;
 }
 
-// [[Rr, Ri], [-Ri, Rr]] per mode, bf16
+// [[Rr, Ri], [-Ri, Rr]] per mode, bf16, and its transpose blkT
 __global__ void __launch_bounds__(256) k_sno_build_blk(const float* __restrict__ Rr, const float* __restrict__ Ri,
-                                                      uint16_t* __restrict__ blk, int K, int C) {
+                                                      uint16_t* __restrict__ blk, uint16_t* __restrict__ blkT, int K,
+                                                      int C) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   const long long per = (long long)C * C;
   if (i >= (long long)K * per) return;
@@ -266,11 +338,12 @@ __global__ void __launch_bounds__(256) k_sno_build_blk(const float* __restrict__
   const int r = (int)((i % per) / C), o = (int)(i % C);
   const float re = Rr[i], im = Ri[i];
   uint16_t* b = blk + k * 4 * per;
+  uint16_t* t = blkT + k * 4 * per;
   const int L = 2 * C;
-  b[r * L + o] = f2bf(re);
-  b[r * L + C + o] = f2bf(im);
-  b[(C + r) * L + o] = f2bf(-im);
-  b[(C + r) * L + C + o] = f2bf(re);
+  b[r * L + o] = t[o * L + r] = f2bf(re);
+  b[r * L + C + o] = t[(C + o) * L + r] = f2bf(im);
+  b[(C + r) * L + o] = t[o * L + C + r] = f2bf(-im);
+  b[(C + r) * L + C + o] = t[(C + o) * L + C + r] = f2bf(re);
 }
 
 // dRr = dBlk[r][o] + dBlk[C+r][C+o]; dRi = dBlk[r][C+o] - dBlk[C+r][o]

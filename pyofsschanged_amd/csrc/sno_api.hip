@@ -27,9 +27,9 @@ struct gnlse_sno {
   SnoLayout lay{};
   long long adam_t = 0;
   DevBuf w, g, m, v;            // FP32 parameters, gradients, Adam moments
-  DevBuf blk, wp16, wq116;      // bf16 operand copies of the weights
+  DevBuf blk, blkT, wp16, wpT16, wq116, wq1T16;  // bf16 operand copies of the weights
   DevBuf Ft, G;                 // bf16 truncated DFT matrices
-  DevBuf H, Z, Vm, Ws, U, Zq, Y1, y;
+  DevBuf H, Hc, Z, Vm, WsT, Zq, Y1, y;
   DevBuf dy, dH, dZ, dZ32, dWm, dVt, dblk, part, dZq, dZq32, rel;
 };
 
@@ -42,12 +42,17 @@ int sno_refresh_bf16(gnlse_sno* s) {
   const long long per = s->C * s->C;
   for (long long l = 0; l < s->L; ++l) {
     hipLaunchKernelGGL(k_sno_build_blk, dim3(blocks_for(s->K * per)), dim3(256), 0, s->stream, W(s, s->lay.Rr[l]),
-                       W(s, s->lay.Ri[l]), (uint16_t*)s->blk.p + l * s->K * 4 * per, (int)s->K, (int)s->C);
+                       W(s, s->lay.Ri[l]), (uint16_t*)s->blk.p + l * s->K * 4 * per,
+                       (uint16_t*)s->blkT.p + l * s->K * 4 * per, (int)s->K, (int)s->C);
     hipLaunchKernelGGL(k_sno_to_bf16, dim3(blocks_for(per)), dim3(256), 0, s->stream, W(s, s->lay.Wp[l]),
                        (uint16_t*)s->wp16.p + l * per, per);
+    hipLaunchKernelGGL(k_sno_to_bf16_t, dim3(blocks_for(per)), dim3(256), 0, s->stream, W(s, s->lay.Wp[l]),
+                       (uint16_t*)s->wpT16.p + l * per, (int)s->C, (int)s->C);
   }
   hipLaunchKernelGGL(k_sno_to_bf16, dim3(blocks_for(s->C * s->CH)), dim3(256), 0, s->stream, W(s, s->lay.Wq1),
                      (uint16_t*)s->wq116.p, s->C * s->CH);
+  hipLaunchKernelGGL(k_sno_to_bf16_t, dim3(blocks_for(s->C * s->CH)), dim3(256), 0, s->stream, W(s, s->lay.Wq1),
+                     (uint16_t*)s->wq1T16.p, (int)s->C, (int)s->CH);
   HIPCHK(hipGetLastError());
   return GNLSE_OK;
 }
@@ -88,36 +93,37 @@ int sno_wgrad(gnlse_sno* s, const uint16_t* A, long long lda, const uint16_t* B,
 int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train) {
   const long long N = s->N, C = s->C, K = s->K, P = B * N;
   uint16_t* H = (uint16_t*)s->H.p;
+  uint16_t* Hc = (uint16_t*)s->Hc.p;
   hipLaunchKernelGGL(k_sno_lift, dim3(blocks_for(P * (C / 8))), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
-                     W(s, s->lay.bl), H, P, (int)s->CI, (int)C);
+                     W(s, s->lay.bl), H, Hc, P, N, (int)s->CI, (int)C);
   HIPCHK(hipGetLastError());
   for (long long l = 0; l < s->L; ++l) {
     const uint16_t* Hl = H + (train ? l : (l & 1)) * P * C;
     uint16_t* Hn = H + (train ? l + 1 : ((l + 1) & 1)) * P * C;
+    const uint16_t* Hcl = Hc + (l & 1) * P * C;
+    uint16_t* Hcn = Hc + ((l + 1) & 1) * P * C;
     uint16_t* Vml = (uint16_t*)s->Vm.p + (train ? l : 0) * K * B * 2 * C;
-    const uint16_t* blk = (const uint16_t*)s->blk.p + l * K * 4 * C * C;
-    // V = DFT_K(H): per field, (2K x N) * (N x C)
-    GEMMCHK((gemm_bf16<128, 64, false, false>(s->stream, (const uint16_t*)s->Ft.p, Hl,
-                                              GemmShape{2 * K, C, N, N, C, 0, N * C, 1}, B, EpiToModes{Vml, B, C})));
-    // Ws = V * [[Rr, Ri], [-Ri, Rr]]: per mode, (B x 2C) * (2C x 2C)
-    GEMMCHK((gemm_bf16<128, 128, false, false>(s->stream, Vml, blk, GemmShape{B, 2 * C, 2 * C, 2 * C, 2 * C, B * 2 * C, 4 * C * C, 1},
-                                               K, EpiToFields{(uint16_t*)s->Ws.p, 2 * K, C})));
-    // U = IDFT_K(Ws): per field, (N x 2K) * (2K x C)
-    GEMMCHK((gemm_bf16<128, 64, false, false>(s->stream, (const uint16_t*)s->G.p, (const uint16_t*)s->Ws.p,
-                                              GemmShape{N, C, 2 * K, 2 * K, C, 0, 2 * K * C, 1}, B,
-                                              EpiIdft{(float*)s->U.p, N, C})));
-    // Z = H Wp + bp + U; H' = gelu(Z)
+    const uint16_t* blkT = (const uint16_t*)s->blkT.p + l * K * 4 * C * C;
+    // V = DFT_K(H): per field, (2K x N) * (N x C) with H channel-first (k-contiguous)
+    GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, (const uint16_t*)s->Ft.p, Hcl,
+                                             GemmShape{2 * K, C, N, N, N, 0, C * N, 1}, B, EpiToModes{Vml, B, C})));
+    // Ws = V * [[Rr, Ri], [-Ri, Rr]]: per mode, (B x 2C) * (2C x 2C), stored per field as [C][2K]
+    GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, Vml, blkT,
+                                              GemmShape{B, 2 * C, 2 * C, 2 * C, 2 * C, B * 2 * C, 4 * C * C, 1}, K,
+                                              EpiToFieldsT{(uint16_t*)s->WsT.p, 2 * K, C})));
+    // Z = IDFT_K(Ws) + H Wp + bp (one GEMM over k = 2K + C); H' = gelu(Z)
     const bool last = (l == s->L - 1);
-    GEMMCHK((gemm_bf16<128, 64, false, false>(
-        s->stream, Hl, (const uint16_t*)s->wp16.p + l * C * C, GemmShape{P, C, C, C, C, 0, 0, 1}, 1,
-        EpiLayer{W(s, s->lay.bp[l]), (const float*)s->U.p, train ? (float*)s->Z.p + l * P * C : nullptr, Hn, C,
-                 last ? 1 : 0})));
+    hipLaunchKernelGGL((k_sno_layer<64>), dim3((unsigned)(N / 128), (unsigned)B), dim3(256), 0, s->stream,
+                       (const uint16_t*)s->G.p, (const uint16_t*)s->WsT.p, Hl,
+                       (const uint16_t*)s->wpT16.p + l * C * C, (const float*)W(s, s->lay.bp[l]),
+                       train ? (float*)s->Z.p + l * P * C : nullptr, Hn, Hcn, N, 2 * K, last ? 1 : 0);
+    HIPCHK(hipGetLastError());
   }
   const uint16_t* HL = H + (train ? s->L : (s->L & 1)) * P * C;
-  GEMMCHK((gemm_bf16<128, 128, false, false>(s->stream, HL, (const uint16_t*)s->wq116.p,
-                                             GemmShape{P, s->CH, C, C, s->CH, 0, 0, 1}, 1,
-                                             EpiProj{W(s, s->lay.bq1), train ? (float*)s->Zq.p : nullptr,
-                                                     (uint16_t*)s->Y1.p, s->CH})));
+  GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, HL, (const uint16_t*)s->wq1T16.p,
+                                            GemmShape{P, s->CH, C, C, C, 0, 0, 1}, 1,
+                                            EpiProj{W(s, s->lay.bq1), train ? (float*)s->Zq.p : nullptr,
+                                                    (uint16_t*)s->Y1.p, s->CH})));
   hipLaunchKernelGGL(k_sno_out, dim3(blocks_for(P * 16)), dim3(256), 0, s->stream, (const uint16_t*)s->Y1.p,
                      W(s, s->lay.Wq2), W(s, s->lay.bq2), x, y, P, (int)s->CH, (int)s->CI);
   HIPCHK(hipGetLastError());
@@ -199,9 +205,10 @@ int gnlse_sno_destroy(gnlse_sno* s) {
   if (!s) return GNLSE_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  DevBuf* bufs[] = {&s->w,  &s->g,  &s->m,  &s->v,  &s->blk, &s->wp16, &s->wq116, &s->Ft,  &s->G,    &s->H,
-                    &s->Z,  &s->Vm, &s->Ws, &s->U,  &s->Zq,  &s->Y1,   &s->y,     &s->dy,  &s->dH,   &s->dZ,
-                    &s->dZ32, &s->dWm, &s->dVt, &s->dblk, &s->part, &s->dZq, &s->dZq32, &s->rel};
+  DevBuf* bufs[] = {&s->w,    &s->g,   &s->m,   &s->v,   &s->blk,  &s->blkT, &s->wp16,  &s->wpT16, &s->wq116,
+                    &s->wq1T16, &s->Ft, &s->G,   &s->H,   &s->Hc,   &s->Z,    &s->Vm,    &s->WsT,   &s->Zq,
+                    &s->Y1,   &s->y,   &s->dy,  &s->dH,  &s->dZ,   &s->dZ32, &s->dWm,   &s->dVt,   &s->dblk,
+                    &s->part, &s->dZq, &s->dZq32, &s->rel};
   for (DevBuf* b : bufs) release(*b);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -214,7 +221,7 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
                      int64_t max_batch, gnlse_sno** out) {
   if (!out) return fail(GNLSE_ERR_ARG, "null out");
   *out = nullptr;
-  if (n < 64 || n % 32 || width < 32 || width % 32 || modes < 16 || modes % 16 || 2 * modes > n || layers < 1 ||
+  if (n < 128 || n % 128 || width != 64 || modes < 16 || modes % 16 || 2 * modes > n || layers < 1 ||
       layers > kSnoMaxLayers || in_channels < 2 || in_channels > 8 || in_channels * width > 256 || hidden < 64 || hidden % 64 ||
       2 * hidden > 256 || max_batch < 1)
     return fail(GNLSE_ERR_ARG, "unsupported surrogate shape");
@@ -262,15 +269,18 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
   al(s->m, o * f4);
   al(s->v, o * f4);
   al(s->blk, L * K * 4 * C * C * h2);
+  al(s->blkT, L * K * 4 * C * C * h2);
   al(s->wp16, L * C * C * h2);
+  al(s->wpT16, L * C * C * h2);
   al(s->wq116, C * CH * h2);
+  al(s->wq1T16, C * CH * h2);
   al(s->Ft, 2 * K * N * h2);
   al(s->G, N * 2 * K * h2);
   al(s->H, (L + 1) * P * C * h2);
+  al(s->Hc, 2 * P * C * h2);
   al(s->Z, L * P * C * f4);
   al(s->Vm, L * K * B * 2 * C * h2);
-  al(s->Ws, B * 2 * K * C * h2);
-  al(s->U, P * C * f4);
+  al(s->WsT, B * 2 * K * C * h2);
   al(s->Zq, P * CH * f4);
   al(s->Y1, P * CH * h2);
   al(s->y, P * 2 * f4);
