@@ -14,9 +14,15 @@ Printed: one JSON line with the metric, the roofline of the propagation
 kernel (algorithmic bytes = 72 FFTs x 32 B x N per attempted step, over the
 kernel's HIP-event duration, against 8 TB/s HBM) and a CPU baseline (the
 NumPy oracle on one host core, timed on a bounded sample of the same
-workload).
+workload). Two further objects report the other halves of the engine:
+"ensemble" (512 noise-seeded soliton realisations on 4096/8192/16384-sample
+grids in one graph-replayed launch with |g12| on the device) and
+"surrogate" (the spectral neural operator trained here on solver-generated
+data, its throughput against the FP64 solver on the same fields, and its
+spectral error after the judge re-solves gated and audited fields).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
+                    [--no-ensemble] [--no-surrogate] [--surrogate-steps S]
 """
 
 import argparse
@@ -43,6 +49,11 @@ def parse():
     ap.add_argument("--length", type=float, default=1.5e-4, help="fibre length in km (default: the full 15 cm)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the noise-ensemble measurement")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
+    ap.add_argument("--surrogate-steps", type=int, default=6000, help="training steps of the surrogate")
+    ap.add_argument("--surrogate-train", type=int, default=20000, help="solver runs in the training set")
+    ap.add_argument("--surrogate-length", type=float, default=2e-3, help="surrogate fibre segment (km)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -107,6 +118,76 @@ def cpu_baseline(A0, length, attempts_full, budget_s):
                    f"({per_attempt * 1e3:.2f} ms/attempt), scaled to the {attempts_full:.0f} attempted steps of "
                    f"a full propagation"),
     }
+
+
+def ensemble_measurement():
+    """512 realisations (128 x 16384, 256 x 8192, 128 x 4096 samples) of the
+    soliton fibre with one-photon-per-mode noise, adaptive RK4IP, one launch
+    replayed from a hipGraph, |g12| per grid on the device."""
+    from pyofsschanged_amd import workloads
+    ens, setups = workloads.noise_ensemble()
+    threads = ens.tune(0)
+    ens.run(graph=True)  # capture + first replay
+    ms = ens.run(graph=True)
+    att = {s["N"]: float(np.mean([r["n_attempts"] for r in ens.output(g)["results"]])) for g, s in enumerate(setups)}
+    total = sum(len(s["seeds"]) for s in setups)
+    print(f"ensemble: {total} realisations in {ms:.0f} ms ({threads} threads per realisation)", file=sys.stderr,
+          flush=True)
+    return {"workload": "soliton fibre (pi/40 km, beta2 -20 ps^2/km, gamma 1.3 /W/km), sech T0 1 ps N=2 + "
+                        "one-photon-per-mode noise, adaptive RK4IP local error 1e-6, 100 ps window",
+            "realisations": {str(s["N"]): len(s["seeds"]) for s in setups},
+            "mean_attempts": {str(k): v for k, v in att.items()},
+            "launch_ms": ms, "propagations_per_s": total / (1e-3 * ms), "threads_per_realisation": threads,
+            "graph": True}
+
+
+def surrogate_measurement(train_count, steps, length):
+    """Train the spectral neural operator on solver-generated propagations,
+    then compare it with the FP64 adaptive solver on 4096 new fields."""
+    from pyofsschanged_amd import surrogate as S
+    t0 = time.perf_counter()
+    prob = S.SegmentProblem(length=length)
+    net = S.SpectralOperator(n=prob.n, max_batch=256)
+    net.init_params(seed=0)
+    train = S.Dataset(prob, net, train_count, seed=1)
+    t1 = time.perf_counter()
+    S.train(net, train, steps, batch=32, lr=1e-3)
+    t_train = time.perf_counter() - t1
+    fields, _ = prob.sample(4096, seed=3)
+    d_in = S.DeviceArray(prob.ctx, fields.shape, np.complex128)
+    d_out = S.DeviceArray(prob.ctx, fields.shape, np.complex128)
+    d_in.upload(fields)
+    t2 = time.perf_counter()
+    res = prob.solve(d_in.ptr, d_out.ptr, len(fields))
+    t_solver = time.perf_counter() - t2
+    ref = d_out.download()
+    sur = S.FibreSurrogate(prob, net, train.scale, gate=0.005, audit_every=64, tol=0.01)
+    sur.infer(d_in.ptr, d_out.ptr, len(fields))  # first call loads code objects
+    t3 = time.perf_counter()
+    sur.infer(d_in.ptr, d_out.ptr, len(fields))
+    t_net = time.perf_counter() - t3
+    raw = S.spectral_error(d_out.download(), ref)
+    rep = sur.propagate(d_in.ptr, d_out.ptr, len(fields), reference=ref)
+    # an ideal judge re-solves exactly the fields above tolerance
+    t_ideal = t_net + t_solver * float(np.mean(raw > 0.01))
+    print(f"surrogate: solver {t_solver:.2f}s, network {t_net:.3f}s, pipeline {rep['t_total_s']:.2f}s, "
+          f"raw err median {np.median(raw):.2e}", file=sys.stderr, flush=True)
+    return {"segment": (f"soliton fibre physics (beta2 -20 ps^2/km, gamma 1.3 /W/km) over {length * 1e3:g} m, "
+                        "4096 samples / 25 ps at 1550 nm; sech 1-10 kW (log-uniform), 30-150 fs FWHM, chirp "
+                        "[-1, 1], one-photon-per-mode noise"),
+            "network": "spectral neural operator, 4 Fourier layers, width 64, 128 modes, bf16 MFMA / FP32 accumulate",
+            "train_fields": train_count, "train_steps": steps, "train_s": t_train,
+            "data_solve_s": train.solve_s, "setup_s": t1 - t0,
+            "fields": len(fields), "solver_s": t_solver,
+            "solver_mean_attempts": float(np.mean([r["n_attempts"] for r in res])),
+            "network_s": t_net, "raw_speedup": t_solver / t_net,
+            "raw_err_median": float(np.median(raw)), "raw_err_p90": float(np.quantile(raw, 0.9)),
+            "raw_within_1pct": float(np.mean(raw <= 0.01)),
+            "pipeline_s": rep["t_total_s"], "speedup": t_solver / rep["t_total_s"],
+            "resolved": rep["resolved"], "within_1pct_after_judge": rep["within_tol"],
+            "err_median_after_judge": rep["err_median"], "err_max_after_judge": rep["err_max"],
+            "speedup_ideal_judge": t_solver / t_ideal, "tol": 0.01, "gate": sur.gate,
+            "audit_every": sur.audit_every}
 
 
 def main():
@@ -205,6 +286,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_measurement()
+    if not args.no_surrogate:
+        line["surrogate"] = surrogate_measurement(args.surrogate_train, args.surrogate_steps, args.surrogate_length)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs[0], args.length, float(np.mean(attempts)) / B, args.cpu_seconds)
     print(json.dumps(line))

@@ -277,15 +277,39 @@ __global__ void __launch_bounds__(256) k_sno_out_bwd(const float* __restrict__ d
   dZq32[i] = g;
 }
 
-// dZ = dH * gelu'(Z) (or dH on the last layer), in bf16 and FP32
+// dZ = dH * gelu'(Z) (or dH on the last layer): bf16 row-major (dZ[p][c]) and
+// channel-first (dZc[b][c][n]) for the GEMMs, FP32 for the bias reduction.
+// Eight channels per thread; consecutive threads hold consecutive points of
+// one channel group.
 __global__ void __launch_bounds__(256) k_sno_act_bwd(const float* __restrict__ dH, const float* __restrict__ Z,
-                                                    uint16_t* __restrict__ dZ, float* __restrict__ dZ32, long long n,
+                                                    uint16_t* __restrict__ dZ, uint16_t* __restrict__ dZc,
+                                                    float* __restrict__ dZ32, long long P, long long N, int C,
                                                     int last) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float g = last ? dH[i] : dH[i] * gelu_grad(Z[i]);
-  dZ[i] = f2bf(g);
-  dZ32[i] = g;
+  const int groups = C / 8;
+  if (i >= P * groups) return;
+  const int g = (int)(i / P);
+  const long long p = i % P;
+  const long long b = p / N, n = p % N;
+  const int c0 = g * 8;
+  const float4* dh4 = reinterpret_cast<const float4*>(dH + p * C + c0);
+  const float4* z4 = reinterpret_cast<const float4*>(Z + p * C + c0);
+  float d[8], z[8];
+  *reinterpret_cast<float4*>(d) = dh4[0];
+  *reinterpret_cast<float4*>(d + 4) = dh4[1];
+  *reinterpret_cast<float4*>(z) = z4[0];
+  *reinterpret_cast<float4*>(z + 4) = z4[1];
+  uint16_t o[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (!last) d[c] *= gelu_grad(z[c]);
+    o[c] = f2bf(d[c]);
+    dZc[(b * C + c0 + c) * N + n] = o[c];
+  }
+  *reinterpret_cast<uint4*>(dZ + p * C + c0) = *reinterpret_cast<const uint4*>(o);
+  float4* out = reinterpret_cast<float4*>(dZ32 + p * C + c0);
+  out[0] = *reinterpret_cast<const float4*>(d);
+  out[1] = *reinterpret_cast<const float4*>(d + 4);
 }
 
 // Partial sums over point chunks of a[p][i] * b[p][j] (a = 1 when null, or

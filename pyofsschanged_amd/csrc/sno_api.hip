@@ -28,9 +28,9 @@ struct gnlse_sno {
   long long adam_t = 0;
   DevBuf w, g, m, v;            // FP32 parameters, gradients, Adam moments
   DevBuf blk, blkT, wp16, wpT16, wq116, wq1T16;  // bf16 operand copies of the weights
-  DevBuf Ft, G;                 // bf16 truncated DFT matrices
+  DevBuf Ft, G, Gt, Fn;         // bf16 truncated DFT matrices (Gt = G^T, Fn = Ft^T)
   DevBuf H, Hc, Z, Vm, WsT, Zq, Y1, y;
-  DevBuf dy, dH, dZ, dZ32, dWm, dVt, dblk, part, dZq, dZq32, rel;
+  DevBuf dy, dH, dZ, dZc, dZ32, dWm, dVt, dblk, part, dZq, dZq32, rel;
 };
 
 namespace {
@@ -151,31 +151,33 @@ int sno_backward(gnlse_sno* s, long long B, const float* x) {
     const uint16_t* Vml = (const uint16_t*)s->Vm.p + l * K * B * 2 * C;
     const uint16_t* blk = (const uint16_t*)s->blk.p + l * K * 4 * C * C;
     uint16_t* dZ = (uint16_t*)s->dZ.p;
-    hipLaunchKernelGGL(k_sno_act_bwd, dim3(blocks_for(P * C)), dim3(256), 0, s->stream, (const float*)dH,
-                       (const float*)s->Z.p + l * P * C, dZ, (float*)s->dZ32.p, P * C, l == s->L - 1 ? 1 : 0);
+    uint16_t* dZc = (uint16_t*)s->dZc.p;
+    hipLaunchKernelGGL(k_sno_act_bwd, dim3(blocks_for(P * (C / 8))), dim3(256), 0, s->stream, (const float*)dH,
+                       (const float*)s->Z.p + l * P * C, dZ, dZc, (float*)s->dZ32.p, P, N, (int)C,
+                       l == s->L - 1 ? 1 : 0);
     HIPCHK(hipGetLastError());
     RET_IF(sno_outer(s, nullptr, nullptr, (const float*)s->dZ32.p, P, 1, (int)C, Gd(s, s->lay.bp[l])));
     RET_IF((sno_wgrad<64, 64>(s, Hl, C, dZ, C, C, C, P, Gd(s, s->lay.Wp[l]))));
     // pointwise path: dH = dZ Wp^T
     GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, dZ, (const uint16_t*)s->wp16.p + l * C * C,
                                              GemmShape{P, C, C, C, C, 0, 0, 1}, 1, EpiF32{dH, C, 0, 0})));
-    // inverse DFT: dWs = G^T dZ per field -> mixing layout
-    GEMMCHK((gemm_bf16<128, 64, true, false>(s->stream, (const uint16_t*)s->G.p, dZ,
-                                             GemmShape{2 * K, C, N, 2 * K, C, 0, N * C, 1}, B,
+    // inverse DFT: dWs = G^T dZ per field -> mixing layout (both operands k-contiguous)
+    GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, (const uint16_t*)s->Gt.p, dZc,
+                                             GemmShape{2 * K, C, N, N, N, 0, C * N, 1}, B,
                                              EpiToModes{(uint16_t*)s->dWm.p, B, C})));
-    // mixing: dV = dWs Blk^T, dBlk = V^T dWs per mode
+    // mixing: dV = dWs Blk^T (stored per field as [C][2K]), dBlk = V^T dWs per mode
     GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, (const uint16_t*)s->dWm.p, blk,
                                               GemmShape{B, 2 * C, 2 * C, 2 * C, 2 * C, B * 2 * C, 4 * C * C, 1}, K,
-                                              EpiToFields{(uint16_t*)s->dVt.p, 2 * K, C})));
+                                              EpiToFieldsT{(uint16_t*)s->dVt.p, 2 * K, C})));
     GEMMCHK((gemm_bf16<128, 128, true, false>(s->stream, Vml, (const uint16_t*)s->dWm.p,
                                               GemmShape{2 * C, 2 * C, B, 2 * C, 2 * C, B * 2 * C, B * 2 * C, 1}, K,
                                               EpiF32{(float*)s->dblk.p, 2 * C, 4 * C * C, 0})));
     hipLaunchKernelGGL(k_sno_fold_blk, dim3(blocks_for(K * C * C)), dim3(256), 0, s->stream, (const float*)s->dblk.p,
                        Gd(s, s->lay.Rr[l]), Gd(s, s->lay.Ri[l]), (int)K, (int)C);
     HIPCHK(hipGetLastError());
-    // DFT: dH += Ft^T dV per field
-    GEMMCHK((gemm_bf16<128, 64, true, false>(s->stream, (const uint16_t*)s->Ft.p, (const uint16_t*)s->dVt.p,
-                                             GemmShape{N, C, 2 * K, N, C, 0, 2 * K * C, 1}, B,
+    // DFT: dH += Ft^T dV per field (Fn = Ft^T, dV as [C][2K])
+    GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, (const uint16_t*)s->Fn.p, (const uint16_t*)s->dVt.p,
+                                             GemmShape{N, C, 2 * K, 2 * K, 2 * K, 0, C * 2 * K, 1}, B,
                                              EpiF32{dH, C, N * C, 1})));
   }
   // lift
@@ -206,8 +208,8 @@ int gnlse_sno_destroy(gnlse_sno* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   DevBuf* bufs[] = {&s->w,    &s->g,   &s->m,   &s->v,   &s->blk,  &s->blkT, &s->wp16,  &s->wpT16, &s->wq116,
-                    &s->wq1T16, &s->Ft, &s->G,   &s->H,   &s->Hc,   &s->Z,    &s->Vm,    &s->WsT,   &s->Zq,
-                    &s->Y1,   &s->y,   &s->dy,  &s->dH,  &s->dZ,   &s->dZ32, &s->dWm,   &s->dVt,   &s->dblk,
+                    &s->wq1T16, &s->Ft, &s->G,   &s->Gt,  &s->Fn,   &s->H,   &s->Hc,   &s->Z,    &s->Vm,    &s->WsT,   &s->Zq,
+                    &s->Y1,   &s->y,   &s->dy,  &s->dH,  &s->dZ,   &s->dZc, &s->dZ32, &s->dWm,   &s->dVt,   &s->dblk,
                     &s->part, &s->dZq, &s->dZq32, &s->rel};
   for (DevBuf* b : bufs) release(*b);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -287,6 +289,9 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
   al(s->dy, P * 2 * f4);
   al(s->dH, P * C * f4);
   al(s->dZ, P * C * h2);
+  al(s->dZc, P * C * h2);
+  al(s->Gt, 2 * K * N * h2);
+  al(s->Fn, N * 2 * K * h2);
   al(s->dZ32, P * C * f4);
   al(s->dWm, K * B * 2 * C * h2);
   al(s->dVt, B * 2 * K * C * h2);
@@ -300,20 +305,22 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
     return rc;
   }
   // truncated DFT matrices from exact angles (reduced mod N in integers)
-  std::vector<uint16_t> ft(2 * K * N), gm(N * 2 * K);
+  std::vector<uint16_t> ft(2 * K * N), gm(N * 2 * K), gt(2 * K * N), fn(N * 2 * K);
   const long double two_pi = 6.283185307179586476925286766559005768L;
   for (long long k = 0; k < K; ++k)
     for (long long j = 0; j < N; ++j) {
       const long double a = two_pi * (long double)((k * j) % N) / (long double)N;
       const double c = (double)cosl(a), sn = (double)sinl(a);
-      ft[(2 * k) * N + j] = f2bf((float)c);
-      ft[(2 * k + 1) * N + j] = f2bf((float)-sn);
+      ft[(2 * k) * N + j] = fn[j * 2 * K + 2 * k] = f2bf((float)c);
+      ft[(2 * k + 1) * N + j] = fn[j * 2 * K + 2 * k + 1] = f2bf((float)-sn);
       const double ck = (k == 0 ? 1.0 : 2.0) / (double)N;
-      gm[j * 2 * K + 2 * k] = f2bf((float)(ck * c));
-      gm[j * 2 * K + 2 * k + 1] = f2bf((float)(-ck * sn));
+      gm[j * 2 * K + 2 * k] = gt[(2 * k) * N + j] = f2bf((float)(ck * c));
+      gm[j * 2 * K + 2 * k + 1] = gt[(2 * k + 1) * N + j] = f2bf((float)(-ck * sn));
     }
   if (upload_raw(device, s->stream, s->Ft, ft.data(), ft.size() * h2) != GNLSE_OK ||
       upload_raw(device, s->stream, s->G, gm.data(), gm.size() * h2) != GNLSE_OK ||
+      upload_raw(device, s->stream, s->Gt, gt.data(), gt.size() * h2) != GNLSE_OK ||
+      upload_raw(device, s->stream, s->Fn, fn.data(), fn.size() * h2) != GNLSE_OK ||
       hipMemsetAsync(s->w.p, 0, o * f4, s->stream) != hipSuccess ||
       hipMemsetAsync(s->m.p, 0, o * f4, s->stream) != hipSuccess ||
       hipMemsetAsync(s->v.p, 0, o * f4, s->stream) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
