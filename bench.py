@@ -51,9 +51,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the noise-ensemble measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
-    ap.add_argument("--surrogate-steps", type=int, default=6000, help="training steps of the surrogate")
+    ap.add_argument("--surrogate-steps", type=int, default=30000, help="training steps of the surrogate")
     ap.add_argument("--surrogate-train", type=int, default=20000, help="solver runs in the training set")
-    ap.add_argument("--surrogate-length", type=float, default=2e-3, help="surrogate fibre segment (km)")
+    ap.add_argument("--surrogate-length", type=float, default=1e-3, help="surrogate fibre segment (km)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -147,7 +147,7 @@ def surrogate_measurement(train_count, steps, length):
     from pyofsschanged_amd import surrogate as S
     t0 = time.perf_counter()
     prob = S.SegmentProblem(length=length)
-    net = S.SpectralOperator(n=prob.n, max_batch=256)
+    net = S.SpectralOperator(n=prob.n, max_batch=1024, in_channels=4)
     net.init_params(seed=0)
     train = S.Dataset(prob, net, train_count, seed=1)
     t1 = time.perf_counter()

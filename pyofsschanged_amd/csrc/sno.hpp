@@ -159,6 +159,60 @@ __global__ void __launch_bounds__(256) k_sno_layer(const uint16_t* __restrict__ 
   }
 }
 
+// Inference head: y = x[..., 0:2] + gelu(H Wq1 + bq1) Wq2 + bq2 for a tile of
+// 128 points, with the CH = 128 hidden channels of the tile held in the MFMA
+// accumulators (never written to memory). Each lane reduces its columns,
+// then the 16 lanes of a row (xor shuffles) and the two column halves of
+// the workgroup (LDS) are combined.
+template <int C, int CH>
+__global__ void __launch_bounds__(256) k_sno_head(const uint16_t* __restrict__ H, const uint16_t* __restrict__ Wq1T,
+                                                 const float* __restrict__ bq1, const float* __restrict__ Wq2,
+                                                 const float* __restrict__ bq2, const float* __restrict__ x,
+                                                 float* __restrict__ y, long long P, int CI) {
+  using Tl = MmaTile<128, CH>;
+  __shared__ __attribute__((aligned(16))) uint16_t As[128 * Tl::LD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[CH * Tl::LD];
+  __shared__ float part[2][128][2];
+  const long long p0 = (long long)blockIdx.x * 128;
+  RowSrc<128> la{H, C, P, p0};
+  RowSrc<CH> lb{Wq1T, C, CH, 0};
+  floatx4_t acc[Tl::TM][Tl::TN];
+  mma_loop<128, CH>(la, lb, 0, C, As, Bs, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+  for (int i = 0; i < Tl::TM; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+      for (int j = 0; j < Tl::TN; ++j) {
+        const int h = wn * Tl::WN + j * 16 + (lane & 15);
+        const float g = gelu_f(acc[i][j][v] + bq1[h]);
+        s0 += g * Wq2[2 * h];
+        s1 += g * Wq2[2 * h + 1];
+      }
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 16);
+        s1 += __shfl_xor(s1, o, 16);
+      }
+      if ((lane & 15) == 0) {
+        const int r = wm * Tl::WM + i * 16 + (lane >> 4) * 4 + v;
+        part[wn][r][0] = s0;
+        part[wn][r][1] = s1;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x < 128) {
+    const long long p = p0 + threadIdx.x;
+    if (p < P) {
+      y[2 * p] = x[p * CI] + bq2[0] + part[0][threadIdx.x][0] + part[1][threadIdx.x][0];
+      y[2 * p + 1] = x[p * CI + 1] + bq2[1] + part[0][threadIdx.x][1] + part[1][threadIdx.x][1];
+    }
+  }
+}
+
 // dst[c][r] = bf16(src[r][c]) for an R x Cc FP32 matrix
 __global__ void __launch_bounds__(256) k_sno_to_bf16_t(const float* __restrict__ src, uint16_t* __restrict__ dst,
                                                       int R, int Cc) {
@@ -169,31 +223,36 @@ __global__ void __launch_bounds__(256) k_sno_to_bf16_t(const float* __restrict__
 }
 
 // ---- elementwise / reductions ---------------------------------------------
-// H0 = x Wl + bl (CI inputs per point), written row-major (Hr[b][n][c], 8
-// channels per thread as one 16-byte store) and channel-first (Hc[b][c][n];
-// consecutive threads hold consecutive points of one channel group, so those
-// 2-byte stores are contiguous across the wavefront).
+// H0 = x Wl + bl (CI inputs per point) for 128 points per workgroup, written
+// row-major (Hr[b][n][c]) and channel-first (Hc[b][c][n]); the channel-first
+// rows are staged in LDS and stored as 16-byte chunks. N % 128 == 0.
+template <int C>
 __global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
                                                  const float* __restrict__ bl, uint16_t* __restrict__ Hr,
-                                                 uint16_t* __restrict__ Hc, long long P, long long N, int CI, int C) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int groups = C / 8;
-  if (i >= P * groups) return;
-  const int g = (int)(i / P);
-  const long long p = i % P;
-  const long long b = p / N, n = p % N;
-  const int c0 = g * 8;
-  float xi[8];
-  for (int j = 0; j < CI; ++j) xi[j] = x[p * CI + j];
-  uint16_t o[8];
+                                                 uint16_t* __restrict__ Hc, long long N, int CI) {
+  constexpr int TP = 128 + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t T[C * TP];
+  const long long p0 = (long long)blockIdx.x * 128;
+  const long long b = p0 / N, n0 = p0 % N;
+  // consecutive threads take consecutive points (conflict-free staging rows)
+  for (int e = threadIdx.x; e < 128 * (C / 8); e += 256) {
+    const int r = e % 128, c0 = (e / 128) * 8;
+    const long long p = p0 + r;
+    uint16_t o[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    float a = bl[c0 + c];
-    for (int j = 0; j < CI; ++j) a += xi[j] * Wl[j * C + c0 + c];
-    o[c] = f2bf(a);
-    Hc[(b * C + c0 + c) * N + n] = o[c];
+    for (int c = 0; c < 8; ++c) {
+      float a = bl[c0 + c];
+      for (int j = 0; j < CI; ++j) a += x[p * CI + j] * Wl[j * C + c0 + c];
+      o[c] = f2bf(a);
+      T[(c0 + c) * TP + r] = o[c];
+    }
+    *reinterpret_cast<uint4*>(Hr + p * C + c0) = *reinterpret_cast<const uint4*>(o);
   }
-  *reinterpret_cast<uint4*>(Hr + p * C + c0) = *reinterpret_cast<const uint4*>(o);
+  __syncthreads();
+  for (int q = threadIdx.x; q < C * 16; q += 256) {
+    const int c = q / 16, c8 = (q % 16) * 8;
+    *reinterpret_cast<uint4*>(Hc + (b * C + c) * N + n0 + c8) = *reinterpret_cast<const uint4*>(&T[c * TP + c8]);
+  }
 }
 
 // y[p][j] = x[p][j] + bq2[j] + sum_h Y1[p][h] Wq2[h][j], j = 0, 1. Sixteen
@@ -349,6 +408,23 @@ __global__ void __launch_bounds__(256) k_sno_sum_parts(const float* __restrict__
   float s = 0.f;
   for (long long k = 0; k < S; ++k) s += part[k * n + i];
   out[i] = s;
+}
+
+// out[i] = sum_s part[s][i] with one workgroup per output element (for many
+// partials of few outputs): strided partial sums, then a fixed-order tree.
+__global__ void __launch_bounds__(256) k_sno_sum_parts_wide(const float* __restrict__ part, float* __restrict__ out,
+                                                           long long S, long long n) {
+  __shared__ float acc[256];
+  const long long i = blockIdx.x;
+  float t = 0.f;
+  for (long long k = threadIdx.x; k < S; k += 256) t += part[k * n + i];
+  acc[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) acc[threadIdx.x] += acc[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = acc[0];
 }
 
 // [[Rr, Ri], [-Ri, Rr]] per mode, bf16, and its transpose blkT

@@ -70,7 +70,7 @@ int sno_outer(gnlse_sno* s, const float* a, const uint16_t* a16, const float* b,
   const long long chunk = (P + nblk - 1) / nblk;
   hipLaunchKernelGGL(k_sno_outer_partial, dim3((unsigned)nblk), dim3(256), 0, s->stream, a, a16, b, (float*)s->part.p,
                      P, NA, NB, chunk);
-  hipLaunchKernelGGL(k_sno_sum_parts, dim3(blocks_for((long long)NA * NB)), dim3(256), 0, s->stream,
+  hipLaunchKernelGGL(k_sno_sum_parts_wide, dim3((unsigned)(NA * NB)), dim3(256), 0, s->stream,
                      (const float*)s->part.p, out, nblk, (long long)NA * NB);
   HIPCHK(hipGetLastError());
   return GNLSE_OK;
@@ -94,8 +94,8 @@ int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train)
   const long long N = s->N, C = s->C, K = s->K, P = B * N;
   uint16_t* H = (uint16_t*)s->H.p;
   uint16_t* Hc = (uint16_t*)s->Hc.p;
-  hipLaunchKernelGGL(k_sno_lift, dim3(blocks_for(P * (C / 8))), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
-                     W(s, s->lay.bl), H, Hc, P, N, (int)s->CI, (int)C);
+  hipLaunchKernelGGL((k_sno_lift<64>), dim3((unsigned)(P / 128)), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
+                     W(s, s->lay.bl), H, Hc, N, (int)s->CI);
   HIPCHK(hipGetLastError());
   for (long long l = 0; l < s->L; ++l) {
     const uint16_t* Hl = H + (train ? l : (l & 1)) * P * C;
@@ -120,6 +120,13 @@ int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train)
     HIPCHK(hipGetLastError());
   }
   const uint16_t* HL = H + (train ? s->L : (s->L & 1)) * P * C;
+  if (!train) {  // hidden layer kept in registers
+    hipLaunchKernelGGL((k_sno_head<64, 128>), dim3((unsigned)((P + 127) / 128)), dim3(256), 0, s->stream, HL,
+                       (const uint16_t*)s->wq1T16.p, W(s, s->lay.bq1), W(s, s->lay.Wq2), W(s, s->lay.bq2), x, y, P,
+                       (int)s->CI);
+    HIPCHK(hipGetLastError());
+    return GNLSE_OK;
+  }
   GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, HL, (const uint16_t*)s->wq1T16.p,
                                             GemmShape{P, s->CH, C, C, C, 0, 0, 1}, 1,
                                             EpiProj{W(s, s->lay.bq1), train ? (float*)s->Zq.p : nullptr,
@@ -225,7 +232,7 @@ int gnlse_sno_create(int device, int64_t n, int width, int modes, int layers, in
   *out = nullptr;
   if (n < 128 || n % 128 || width != 64 || modes < 16 || modes % 16 || 2 * modes > n || layers < 1 ||
       layers > kSnoMaxLayers || in_channels < 2 || in_channels > 8 || in_channels * width > 256 || hidden < 64 || hidden % 64 ||
-      2 * hidden > 256 || max_batch < 1)
+      2 * hidden > 256 || hidden != 128 || max_batch < 1)
     return fail(GNLSE_ERR_ARG, "unsupported surrogate shape");
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)

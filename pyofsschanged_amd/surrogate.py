@@ -4,15 +4,15 @@ solver as its judge and fallback.
 
 Problem (fixed for one trained network): the soliton fibre's physics
 (beta2 = -20 ps^2/km, gamma = 1.3 /W/km, Kerr) over a segment of length L
-(default 2 m) on 4096 samples over a 25 ps window at 1550 nm. The input
+(default 1 m) on 4096 samples over a 25 ps window at 1550 nm. The input
 family: sech pulses with peak power log-uniform in [1, 10] kW, FWHM uniform
 in [30, 150] fs, chirp uniform in [-1, 1], each with seeded one-photon-per-
 mode noise. (At the 100 ps / 4096-sample grid of the 1000-step soliton, dt =
 24.4 fs would not resolve a 30 fs pulse, and over the whole soliton period
 such pulses disperse beyond any window of 4096 samples that resolves them.
-With dt = 6.1 fs and L = 2 m the spectra stay within the band and at most
+With dt = 6.1 fs and L <= 2 m the spectra stay within the band and at most
 4e-4 of the power reaches the outer tenth of the window, while the solver
-still needs 22-617 attempted steps per field.)
+still needs about 200 (1 m) to 360 (2 m) attempted steps per field.)
 
 Representation: the network predicts the output in the interaction picture
 of the linear operator D (whose action over L is known exactly),
@@ -188,7 +188,7 @@ class SegmentProblem:
     """The fibre segment the surrogate stands in for, its input family, and
     the device-side encoding of fields."""
 
-    def __init__(self, samples=4096, window=25.0, wavelength=1550.0, length=2e-3, beta=(0.0, 0.0, -20.0),
+    def __init__(self, samples=4096, window=25.0, wavelength=1550.0, length=1e-3, beta=(0.0, 0.0, -20.0),
                  gamma=1.3, local_error=1e-6, time_domain=True):
         self.domain = Domain(1, samples, window, C_NM_PS / wavelength)
         self.length = length
@@ -320,7 +320,7 @@ class FibreSurrogate:
     """Network inference for whole batches of fields on the device, judged
     by the FP64 solver (energy gate + audit) with re-solving as fallback."""
 
-    def __init__(self, problem, net, scale, gate=0.02, audit_every=64, tol=0.01, chunk=None):
+    def __init__(self, problem, net, scale, gate=0.005, audit_every=64, tol=0.01, chunk=None):
         self.problem, self.net, self.scale = problem, net, scale
         self.gate, self.audit_every, self.tol = gate, audit_every, tol
         self.chunk = chunk or net.max_batch
