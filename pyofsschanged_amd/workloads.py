@@ -9,6 +9,11 @@
                 examples/dudley_sc/fig_3.py:35-36), self-steepening and
                 Blow-Wood Raman, 8192 samples over 12.5 ps, adaptive RK4IP with
                 local error 1e-6.
+  amplifier     saturable-gain amplifier at 1030 nm: 200 fs FWHM Gaussian of
+                1 nJ (4697.2 W) with linear chirp C, 5 m, beta2 = 0.023 ps^2/m,
+                gamma = 5e-3 /W/m, Raman (use_all), 15 dB small-signal gain,
+                E_sat = 5 nJ, Lorentzian gain profile 40 nm wide, 16384
+                samples over 40 ps, adaptive RK4IP.
   noise_ensemble  realisations of the soliton fibre (adaptive RK4IP, local
                 error 1e-6) on several grids over the same 100 ps window
                 (default 128 x 16384, 256 x 8192, 128 x 4096 samples), each
@@ -29,6 +34,7 @@ from .domain import Domain
 from .field import fftshift, ifft
 from ._lib import Ensemble
 from .modules.fibre import Fibre
+from .modules.gaussian import Gaussian
 from .modules.sech import Sech
 
 C_NM_PS = Domain.vacuum_light_speed
@@ -62,6 +68,18 @@ def supercontinuum(samples=8192, length=1.5e-4, local_error=1e-6, traces=1):
     fibre = Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
                   method="ark4ip", local_error=local_error, traces=traces)
     return domain, pulse, fibre
+
+
+def amplifier(samples=16384, chirps=None, length=5e-3, local_error=1e-6):
+    """Domain, input fields (one per chirp value) and Fibre of the amplifier
+    batch; chirps default to 64 values evenly spaced in [-2, 2]."""
+    chirps = np.linspace(-2.0, 2.0, 64) if chirps is None else np.asarray(chirps, dtype=float)
+    domain = Domain(1, samples, 40.0, C_NM_PS / 1030.0)
+    fields = np.stack([Gaussian(peak_power=4697.2, width=0.2, using_fwhm=True, C=float(c)).generate(domain.t)
+                       for c in chirps])
+    fibre = Fibre(length=length, beta=[0.0, 0.0, 23.0], gamma=5.0, use_all=True, small_signal_gain=15.0,
+                  E_sat=5.0, lamb0=1030.0, bandwidth=40.0, method="ark4ip", local_error=local_error)
+    return domain, fields, fibre
 
 
 NOISE_ENSEMBLE_GROUPS = ((16384, 128), (8192, 256), (4096, 128))
