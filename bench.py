@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the noise-ensemble measurement")
     ap.add_argument("--no-amplifier", action="store_true", help="skip the amplifier-batch measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
-    ap.add_argument("--surrogate-steps", type=int, default=30000, help="training steps of the surrogate")
+    ap.add_argument("--surrogate-steps", type=int, default=60000, help="training steps of the surrogate")
     ap.add_argument("--surrogate-train", type=int, default=20000, help="solver runs in the training set")
     ap.add_argument("--surrogate-length", type=float, default=1e-3, help="surrogate fibre segment (km)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

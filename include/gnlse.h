@@ -168,6 +168,7 @@ int gnlse_malloc(gnlse_ctx* ctx, int64_t bytes, void** dptr);
 int gnlse_free(gnlse_ctx* ctx, void* dptr);
 int gnlse_memcpy_htod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int gnlse_memcpy_dtoh(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int gnlse_memcpy_dtod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results);
 

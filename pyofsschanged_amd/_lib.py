@@ -98,6 +98,7 @@ SIGNATURES = {
     "gnlse_free": [_P, _P],
     "gnlse_memcpy_htod": [_P, _P, _P, _I64],
     "gnlse_memcpy_dtoh": [_P, _P, _P, _I64],
+    "gnlse_memcpy_dtod": [_P, _P, _P, _I64],
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
@@ -338,6 +339,9 @@ class Context:
 
     def dtoh(self, host, dptr):
         self._call("gnlse_memcpy_dtoh", _ptr(host), dptr, host.nbytes)
+
+    def dtod(self, dst, src, nbytes):
+        self._call("gnlse_memcpy_dtod", dst, src, int(nbytes))
 
     def propagate_adaptive_device(self, method, batch, params, d_in, d_out):
         res = (InstanceResult * int(batch))()

@@ -428,6 +428,14 @@ int gnlse_memcpy_dtoh(gnlse_ctx* c, void* dst, const void* src, int64_t bytes) {
   return GNLSE_OK;
 }
 
+int gnlse_memcpy_dtod(gnlse_ctx* c, void* dst, const void* src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return fail(GNLSE_ERR_ARG, "invalid copy arguments");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
 int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results) {
   RET_IF(check_ready(c, true, true));

@@ -49,6 +49,15 @@ struct EpiToModes {
     out[((row >> 1) * B + b) * 2 * C + (row & 1) * C + col] = f2bf(v);
   }
 };
+// DFT of all fields as one GEMM (row = 2k + re/im, col = b*C + c) -> [k][b][2C]
+struct EpiToModesAll {
+  uint16_t* out;
+  long long B, C;
+  __device__ void operator()(int, long long row, long long col, float v) const {
+    const long long b = col / C, c = col % C;
+    out[((row >> 1) * B + b) * 2 * C + (row & 1) * C + c] = f2bf(v);
+  }
+};
 // mixing (batch = mode k, row = field b, col = j in [0, 2C)) -> [b][2K][C]
 struct EpiToFields {
   uint16_t* out;
@@ -223,33 +232,36 @@ __global__ void __launch_bounds__(256) k_sno_to_bf16_t(const float* __restrict__
 }
 
 // ---- elementwise / reductions ---------------------------------------------
-// H0 = x Wl + bl (CI inputs per point) for 128 points per workgroup, written
-// row-major (Hr[b][n][c]) and channel-first (Hc[b][c][n]); the channel-first
-// rows are staged in LDS and stored as 16-byte chunks. N % 128 == 0.
+// H0 = x Wl + bl (CI inputs per point) for 128 points per workgroup (one
+// point per thread): each thread writes its point's C channels as one
+// contiguous row (Hr[b][n][c]) and stages them in LDS, from where whole
+// channel-first rows (Hc[b][c][n]) are stored as 16-byte chunks. N % 128 == 0.
 template <int C>
-__global__ void __launch_bounds__(256) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
+__global__ void __launch_bounds__(128) k_sno_lift(const float* __restrict__ x, const float* __restrict__ Wl,
                                                  const float* __restrict__ bl, uint16_t* __restrict__ Hr,
                                                  uint16_t* __restrict__ Hc, long long N, int CI) {
   constexpr int TP = 128 + 8;
   __shared__ __attribute__((aligned(16))) uint16_t T[C * TP];
   const long long p0 = (long long)blockIdx.x * 128;
   const long long b = p0 / N, n0 = p0 % N;
-  // consecutive threads take consecutive points (conflict-free staging rows)
-  for (int e = threadIdx.x; e < 128 * (C / 8); e += 256) {
-    const int r = e % 128, c0 = (e / 128) * 8;
-    const long long p = p0 + r;
+  const int r = threadIdx.x;
+  const long long p = p0 + r;
+  float xi[8];
+  for (int j = 0; j < CI; ++j) xi[j] = x[p * CI + j];
+#pragma unroll
+  for (int c0 = 0; c0 < C; c0 += 8) {
     uint16_t o[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       float a = bl[c0 + c];
-      for (int j = 0; j < CI; ++j) a += x[p * CI + j] * Wl[j * C + c0 + c];
+      for (int j = 0; j < CI; ++j) a += xi[j] * Wl[j * C + c0 + c];
       o[c] = f2bf(a);
       T[(c0 + c) * TP + r] = o[c];
     }
     *reinterpret_cast<uint4*>(Hr + p * C + c0) = *reinterpret_cast<const uint4*>(o);
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < C * 16; q += 256) {
+  for (int q = threadIdx.x; q < C * 16; q += 128) {
     const int c = q / 16, c8 = (q % 16) * 8;
     *reinterpret_cast<uint4*>(Hc + (b * C + c) * N + n0 + c8) = *reinterpret_cast<const uint4*>(&T[c * TP + c8]);
   }

@@ -94,7 +94,7 @@ int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train)
   const long long N = s->N, C = s->C, K = s->K, P = B * N;
   uint16_t* H = (uint16_t*)s->H.p;
   uint16_t* Hc = (uint16_t*)s->Hc.p;
-  hipLaunchKernelGGL((k_sno_lift<64>), dim3((unsigned)(P / 128)), dim3(256), 0, s->stream, x, W(s, s->lay.Wl),
+  hipLaunchKernelGGL((k_sno_lift<64>), dim3((unsigned)(P / 128)), dim3(128), 0, s->stream, x, W(s, s->lay.Wl),
                      W(s, s->lay.bl), H, Hc, N, (int)s->CI);
   HIPCHK(hipGetLastError());
   for (long long l = 0; l < s->L; ++l) {
@@ -104,9 +104,11 @@ int sno_forward(gnlse_sno* s, long long B, const float* x, float* y, bool train)
     uint16_t* Hcn = Hc + ((l + 1) & 1) * P * C;
     uint16_t* Vml = (uint16_t*)s->Vm.p + (train ? l : 0) * K * B * 2 * C;
     const uint16_t* blkT = (const uint16_t*)s->blkT.p + l * K * 4 * C * C;
-    // V = DFT_K(H): per field, (2K x N) * (N x C) with H channel-first (k-contiguous)
-    GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, (const uint16_t*)s->Ft.p, Hcl,
-                                             GemmShape{2 * K, C, N, N, N, 0, C * N, 1}, B, EpiToModes{Vml, B, C})));
+    // V = DFT_K(H) of all fields as one GEMM: (2K x N) * (N x B*C), the
+    // channel-first activations being a k-contiguous (B*C) x N matrix
+    GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, (const uint16_t*)s->Ft.p, Hcl,
+                                              GemmShape{2 * K, B * C, N, N, N, 0, 0, 1}, 1,
+                                              EpiToModesAll{Vml, B, C})));
     // Ws = V * [[Rr, Ri], [-Ri, Rr]]: per mode, (B x 2C) * (2C x 2C), stored per field as [C][2K]
     GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, Vml, blkT,
                                               GemmShape{B, 2 * C, 2 * C, 2 * C, 2 * C, B * 2 * C, 4 * C * C, 1}, K,
@@ -169,9 +171,9 @@ int sno_backward(gnlse_sno* s, long long B, const float* x) {
     GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, dZ, (const uint16_t*)s->wp16.p + l * C * C,
                                              GemmShape{P, C, C, C, C, 0, 0, 1}, 1, EpiF32{dH, C, 0, 0})));
     // inverse DFT: dWs = G^T dZ per field -> mixing layout (both operands k-contiguous)
-    GEMMCHK((gemm_bf16<128, 64, false, true>(s->stream, (const uint16_t*)s->Gt.p, dZc,
-                                             GemmShape{2 * K, C, N, N, N, 0, C * N, 1}, B,
-                                             EpiToModes{(uint16_t*)s->dWm.p, B, C})));
+    GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, (const uint16_t*)s->Gt.p, dZc,
+                                              GemmShape{2 * K, B * C, N, N, N, 0, 0, 1}, 1,
+                                              EpiToModesAll{(uint16_t*)s->dWm.p, B, C})));
     // mixing: dV = dWs Blk^T (stored per field as [C][2K]), dBlk = V^T dWs per mode
     GEMMCHK((gemm_bf16<128, 128, false, true>(s->stream, (const uint16_t*)s->dWm.p, blk,
                                               GemmShape{B, 2 * C, 2 * C, 2 * C, 2 * C, B * 2 * C, 4 * C * C, 1}, K,
