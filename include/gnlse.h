@@ -172,6 +172,19 @@ int gnlse_memcpy_dtod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes)
 int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results);
 
+/* Trace characteristics, Storage.update_characts (pyofss/modules/storage.py:
+ * 194-209) for `count` fields A (host, complex128, time domain) in one
+ * launch. out[4k..4k+3] = energy (sum of simpson_w * |A|^2, simpson_w the
+ * old-Simpson weights of t * 1e-3, field.py:161-172), maximum power, width
+ * of the highest temporal peak at half prominence in samples
+ * (get_duration, field.py:207-229, 283-287) and spectral width in samples
+ * (get_duration_spec, field.py:232-280; NaN where no spectral peak has
+ * prominence >= max/100). n_peaks[k] and peaks[k * max_peaks ...] = the
+ * temporal peaks with prominence >= max/10 in increasing order (get_peaks,
+ * field.py:272-274; at most max_peaks stored, n_peaks counts all). */
+int gnlse_trace_characteristics(gnlse_ctx* ctx, int64_t count, const double* A, const double* simpson_w, double* out,
+                                int32_t* n_peaks, int32_t* peaks, int max_peaks);
+
 /* Duration in ms of the last propagation kernel, from HIP events recorded on
  * the context's stream around the launch. */
 int gnlse_last_kernel_ms(gnlse_ctx* ctx, double* ms);

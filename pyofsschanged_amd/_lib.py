@@ -101,6 +101,7 @@ SIGNATURES = {
     "gnlse_memcpy_dtod": [_P, _P, _P, _I64],
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
     "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
     "gnlse_ensemble_tune": [_P, _INT, _INT, ctypes.POINTER(ctypes.c_int)],
     "gnlse_ensemble_threads": [_P, ctypes.POINTER(ctypes.c_int)],
@@ -353,6 +354,22 @@ class Context:
         v = ctypes.c_double()
         self._call("gnlse_last_kernel_ms", ctypes.byref(v))
         return v.value
+
+    def trace_characteristics(self, A, simpson_w, max_peaks=256):
+        """Storage.update_characts quantities of every row of A on the device:
+        dict of energy, max_power, duration (samples), spec_width (samples,
+        NaN without a qualifying spectral peak), peaks (list of index arrays)."""
+        a, _ = self._batched(A)
+        w = _f64(simpson_w, self.n)
+        k = a.shape[0]
+        out = np.empty((k, 4), dtype=np.float64)
+        npk = np.empty(k, dtype=np.int32)
+        pk = np.empty((k, max_peaks), dtype=np.int32)
+        self._call("gnlse_trace_characteristics", k, _ptr(a), _ptr(w), _ptr(out), _ptr(npk), _ptr(pk), int(max_peaks))
+        if np.any(npk > max_peaks):
+            raise GNLSEError(f"more than {max_peaks} temporal peaks in a trace")
+        return {"energy": out[:, 0], "max_power": out[:, 1], "duration": out[:, 2], "spec_width": out[:, 3],
+                "peaks": [pk[i, :npk[i]].astype(np.int64) for i in range(k)]}
 
 
 def _results(res):

@@ -108,7 +108,8 @@ int gnlse_ctx_destroy(gnlse_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->lin,  &c->efix,  &c->nlmul, &c->hr,   &c->omega, &c->gprof, &c->simpw, &c->tw,
                     &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
-                    &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr};
+                    &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr,
+                    &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk};
   for (DevBuf* b : bufs) release(*b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -449,6 +450,37 @@ int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, con
   RET_IF(launch_adaptive(c, method, batch, prm, (const double2*)d_A_in, (double2*)d_A_out));
   HIPCHK(hipMemcpyAsync(results, c->res.p, (size_t)batch * sizeof(gnlse_instance_result), hipMemcpyDeviceToHost,
                         c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_trace_characteristics(gnlse_ctx* c, int64_t count, const double* A, const double* simpson_w, double* out,
+                                int32_t* n_peaks, int32_t* peaks, int max_peaks) {
+  RET_IF(check_ready(c, false, false));
+  if (count < 1 || count > 0x7fffffff || !A || !simpson_w || !out || !n_peaks || (max_peaks > 0 && !peaks) ||
+      max_peaks < 0)
+    return fail(GNLSE_ERR_ARG, "invalid characteristics arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t fb = (size_t)count * c->n * sizeof(double2);
+  RET_IF(io_prepare(c, count, A, c->io_in));
+  RET_IF(ensure(c->io_b, fb));
+  RET_IF(ensure(c->ch_out, (size_t)count * 4 * sizeof(double)));
+  RET_IF(ensure(c->ch_npk, (size_t)count * sizeof(int)));
+  RET_IF(ensure(c->ch_pk, (size_t)count * (max_peaks > 0 ? max_peaks : 1) * sizeof(int)));
+  RET_IF(ensure(c->ch_w, (size_t)c->n * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(c->ch_w.p, simpson_w, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // pyofss spectra (field.fft), then the characteristics of every trace
+  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p);
+  if (e == hipSuccess)
+    e = c->tab->characts(c->stream, count, (const double2*)c->io_in.p, (const double2*)c->io_b.p,
+                         (const double*)c->ch_w.p, (double*)c->ch_out.p, (int*)c->ch_npk.p, (int*)c->ch_pk.p,
+                         max_peaks);
+  if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("characteristics: ") + hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(out, c->ch_out.p, (size_t)count * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(n_peaks, c->ch_npk.p, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (max_peaks > 0)
+    HIPCHK(hipMemcpyAsync(peaks, c->ch_pk.p, (size_t)count * max_peaks * sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }

@@ -53,6 +53,7 @@ struct gnlse_ctx {
   bool have_lin = false, have_nl = false;
   Ops ops{};
   DevBuf ws, io_in, io_out, io_b, tr_A, tr_z, st_z, st_h, lg_h, lg_d, lg_a, rp_h, rp_a, res, ntr;
+  DevBuf ch_w, ch_out, ch_npk, ch_pk;  // trace characteristics
 };
 
 inline int ensure(DevBuf& b, size_t bytes) {

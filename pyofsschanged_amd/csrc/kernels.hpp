@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/gnlse.h"
+#include "characts.hpp"
 #include "ops.hpp"
 
 namespace gn {
@@ -54,6 +55,8 @@ struct LaunchTable {
                       double2* out, double2* ws);
   hipError_t (*adaptive)(hipStream_t, const Ops&, int method, const AdaptiveParams&, long long batch,
                          const double2* in, double2* out, double2* ws);
+  hipError_t (*characts)(hipStream_t, long long count, const double2* A, const double2* S, const double* w,
+                         double* out, int* npk, int* pk, int max_pk);
 };
 
 const LaunchTable* find_table(long long n);
@@ -391,7 +394,13 @@ struct Launcher {
     else GN_BY_KIND(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
-  static constexpr LaunchTable table = {N, T, &fft, &make_exp, &linear, &linfac, &nonlin, &exp_nonlin, &step, &fixed, &adaptive};
+  static hipError_t characts(hipStream_t s, long long count, const double2* A, const double2* S, const double* w,
+                             double* out, int* npk, int* pk, int max_pk) {
+    hipLaunchKernelGGL((k_characts<N>), dim3((unsigned)count), dim3(kCharT), 0, s, A, S, w, out, npk, pk, max_pk);
+    return hipGetLastError();
+  }
+  static constexpr LaunchTable table = {N, T, &fft, &make_exp, &linear, &linfac, &nonlin, &exp_nonlin, &step, &fixed,
+                                        &adaptive, &characts};
 };
 
 }  // namespace gn

@@ -69,12 +69,14 @@ class Fibre:
                 return self.l(A, z) + self.n(A, z)
 
         class FunctionCharacts:
-            def __init__(self, l_d, l_nl):
+            def __init__(self, l_d, l_nl, lengths):
                 self.l_d = l_d
                 self.l_nl = l_nl
+                self.lengths = lengths
 
         self.function_step = FunctionStep(self.l, self.n, self.linear, self.nonlinear, self.amplifier_step)
-        self.function_characts = FunctionCharacts(self.get_dispersion_length, self.get_nonlinear_length)
+        self.function_characts = FunctionCharacts(self.get_dispersion_length, self.get_nonlinear_length,
+                                                  self.lengths_from)
         self.stepper = Stepper(traces, local_error, method, self.function_step, self.function_characts, self.length,
                                total_steps, dir, save_represent, cycle=self.cycle, fibre_name=self.name,
                                downsampling=downsampling)
@@ -145,6 +147,14 @@ class Fibre:
             self.reference_length = None
         else:
             self.reference_length = self.L_NL if self.L_D is None else self.L_D
+
+    def lengths_from(self, T_0, P_0):
+        """(L_D, L_NL) of fibre.py:179-187 from a trace's duration T_0 and
+        maximum power P_0."""
+        with np.errstate(divide="ignore"):
+            L_NL = 1 / (self.gamma * np.float64(P_0))
+            L_D = None if self.beta_2 is None else T_0 ** 2 / abs(np.float64(self.beta_2))
+        return L_D, L_NL
 
     def get_nonlinear_length(self, field_in):
         """fibre.py:179-182"""

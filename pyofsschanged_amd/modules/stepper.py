@@ -142,8 +142,7 @@ class Stepper:
                                              self.total_steps, trace_every, len(steps),
                                              h_exp=fibre.linear_step(0.5 * h))
         fibre._count_ffts(self.solver.method_id, self.total_steps)
-        for k, i in enumerate(steps):
-            self.storage.append(zs[i] + h, tr[0, k].copy())
+        self.storage.append_traces([zs[i] + h for i in steps], [tr[0, k].copy() for k in range(len(steps))], ctx)
         self.storage.store_current_fft_count()
         self.A_out = A_out
         self.save_df()
@@ -193,11 +192,14 @@ class Stepper:
         self.attempts = res["n_attempts"]
         self.accepted = res["n_accepted"]
         fibre._count_ffts(self.solver.method_id, 3 * res["n_attempts"])
+        zs, traces = [], []
         for k in range(res["n_traces"]):
             z = float(out["trace_z"][0, k])
             if k > 0:  # stepper.py:343: step sizes only with the traces after z = 0
                 self.storage.step_sizes.append((z, float(out["step_h"][0, k])))
-            self.storage.append(z, out["traces"][0, k].copy())
+            zs.append(z)
+            traces.append(out["traces"][0, k].copy())
+        self.storage.append_traces(zs, traces, ctx)
         raise_for_status(res["status"])
         self.storage.store_current_fft_count()
         self.A_out = out["A"]

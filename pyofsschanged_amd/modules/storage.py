@@ -2,8 +2,9 @@
 
 Traces come back from the device as distinct arrays (the reference keeps a
 reference to each stored field, storage.py:190-191). The per-trace
-characteristics of update_characts (storage.py:194-209) are host-side
-diagnostics computed from the returned traces.
+characteristics of update_characts (storage.py:194-209) of a whole
+propagation are computed on the device in one launch (append_traces);
+append keeps the reference's one-trace form for direct callers.
 """
 
 import os
@@ -13,7 +14,8 @@ import numpy as np
 import pandas as pd
 
 from .. import field
-from ..field import energy, get_duration, get_duration_spec, get_peaks, spectral_power, temporal_power
+from ..field import (energy, get_duration, get_duration_spec, get_peaks, simpson_weights, spectral_power,
+                     temporal_power)
 
 
 class StorageError(Exception):
@@ -93,6 +95,32 @@ class Storage:
         self.z.append(z)
         self.As.append(A)
         self.update_characts(A)
+
+    def append_traces(self, zs, As, ctx):
+        """storage.py:183-209 for all traces of a propagation: z and field
+        are stored as by append, and the characteristics (energy, maximum
+        power, peaks, duration, spectral width, L_D, L_NL) of every trace come
+        from one device launch on the context's grid. Differences from the
+        reference: a trace without a spectral peak of prominence >= max/100
+        gets a NaN spectral width (the reference fails on abs(None),
+        field.py:278) and one without any temporal peak a NaN duration."""
+        self.z.extend(zs)
+        self.As.extend(As)
+        if self.domain is None or self.f is None or len(As) == 0:
+            return
+        ch = ctx.trace_characteristics(np.stack(As), simpson_weights(self.domain.t * 1e-3))
+        field.fft_counter += len(As)  # spectral_power's field.fft per trace (storage.py:203)
+        for k in range(len(As)):
+            T0 = float(ch["duration"][k]) * self.domain.dt
+            P0 = float(ch["max_power"][k])
+            self.energy_list.append(float(ch["energy"][k]))
+            self.max_power_list.append(P0)
+            self.peaks_list.append(ch["peaks"][k])
+            self.duration_list.append(T0)
+            self.spec_width_list.append(float(ch["spec_width"][k]) * self.domain.dnu)
+            L_D, L_NL = self.f.lengths(T0, P0)
+            self.l_d_list.append(L_D)
+            self.l_nl_list.append(L_NL)
 
     def update_characts(self, A):
         """storage.py:194-209 (skipped without a domain or length functions)."""
