@@ -202,6 +202,8 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   d.load(x, in + b * N);
   d.store(sA, x);
 
+  double2* sNA = ws + S_NA * N;
+  bool nA_valid = false;  // sNA holds N(sA)
   double z = 0.0, h = prm.h0, z_ignore = prm.step_storage;
   long long n_tr = 0, n_att = 0, n_acc = 0;
   int status = GNLSE_STATUS_OK;
@@ -242,7 +244,10 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       for (int c = 0; c < 3; ++c) {
         const double2* src = (c == 2) ? sH : sA;
         const double2* E = (c == 0) ? sEH : sEQ;
-        integrator<METHOD>(d, x, src, E, c == 0 ? h : h_half, ws);
+        // N(A) is shared by the coarse and the first half step, and by every
+        // attempt from the same accepted field
+        integrator<METHOD>(d, x, src, E, c == 0 ? h : h_half, ws, sNA, c < 2 && nA_valid, c < 2 && !nA_valid);
+        if (c < 2) nA_valid = true;
         if (c == 0) d.store(sC, x);
         else if (c == 1) d.store(sH, x);
       }
@@ -287,6 +292,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         for (int k = 0; k < P; ++k)
           x[k] = csub(cscale(x[k], prm.f_alpha), cscale(sC[d.idx(k)], prm.f_beta));
         d.store(sA, x);
+        nA_valid = false;
         if (z > z_ignore) {  // stepper.py:342-345
           if (prm.max_traces > 0 && !store_trace(z, h_temp)) status = GNLSE_STATUS_TRACE_OVERFLOW;
           z_ignore += prm.step_storage;

@@ -340,13 +340,16 @@ struct Dev {
 };
 
 // Per-instance scratch slots (N complex each).
-enum Slot { S_A = 0, S_C, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, NSLOT };
+enum Slot { S_A = 0, S_C, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, S_NA, NSLOT };
 
 // solver.py:339-354 rk4ip as a 7-stage loop (one call site per operator).
-// Input field in slot `in`; E = exp(factor*h/2). Result in x.
+// Input field in slot `in`; E = exp(factor*h/2). Result in x. N(in) is read
+// from nA instead of being computed when nA_ready, and stored to nA when
+// nA_store (the step-doubling controller evaluates N of the same field for
+// its coarse step and its first half step).
 template <int N, int T, int K>
 __device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
-                      double2* ws) {
+                      double2* ws, double2* nA = nullptr, bool nA_ready = false, bool nA_store = false) {
   constexpr int P = N / T;
   double2* sAI = ws + S_AI * N;
   double2* sK0 = ws + S_K0 * N;
@@ -357,7 +360,7 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], cons
 #pragma unroll 1
   for (int st = 0; st < 7; ++st) {
     // pre
-    if (st == 0 || st == 2) d.load(x, in);
+    if ((st == 0 && !nA_ready) || st == 2) d.load(x, in);
     else if (st == 3) {  // A_I + 0.5*k0
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = cadd(x[k], cscale(sK0[d.idx(k)], 0.5));
@@ -367,7 +370,12 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], cons
     } else if (st == 6) d.load(x, sK0);
     if (st == 1 || st == 2 || st == 5 || st == 6) d.linear(x, E, hh);
     if (st == 0 || st == 3 || st == 4 || st == 5) {
-      d.nonlin(x, sU, sT);
+      if (st == 0 && nA_ready) {
+        d.load(x, nA);
+      } else {
+        d.nonlin(x, sU, sT);
+        if (st == 0 && nA_store) d.store(nA, x);
+      }
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
     }
@@ -413,8 +421,9 @@ __device__ __forceinline__ void ss_symmetric(Dev<N, T, K>& d, double2 (&x)[N / T
 
 template <int METHOD, int N, int T, int K>
 __device__ __forceinline__ void integrator(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E,
-                                           double h, double2* ws) {
-  if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws);
+                                           double h, double2* ws, double2* nA = nullptr, bool nA_ready = false,
+                                           bool nA_store = false) {
+  if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);
   else ss_symmetric(d, x, in, E, h, ws);
 }
 
