@@ -9,6 +9,7 @@ Restates, in complex128, the arithmetic of these reference files
   pyofss/modules/linearity.py:35-272                  linear operator
   pyofss/modules/nonlinearity.py:37-270               nonlinear operator
   pyofss/modules/amplifier.py:42-175                  saturable gain
+  pyofss/modules/amplifier.py:178-402                 two-level Yb gain (CPU path)
   pyofss/modules/solver.py:24-354                     integrators
   pyofss/modules/stepper.py:33-376                    fixed and adaptive steppers
   pyofss/modules/storage.py:130-192                   trace capture
@@ -39,17 +40,22 @@ Pins (tests/test_oracle_*.py):
   adaptive stepper            -> controller arithmetic replayed by hand on a tiny
                                  case; convergence to the closed-form soliton
   amplifier                   -> small-signal gain closed form
+  two-level Yb gain           -> the df_results values printed by
+                                 examples/notebooks/nonlinear_attractor.ipynb
+                                 (first five traces, 1e-11; tests/test_yb_oracle.py),
+                                 pump-only inversion closed form
+  Hollenbeck-Cantrell Raman   -> the same notebook run (its use_all='hollenbeck')
   simps_avg                   -> exact on cubics, reduces to scipy.simpson on odd N
   OPPM noise, coherence       -> Parseval, |g12| closed forms (build-only parts)
-Parity unpinned: Hollenbeck-Cantrell normalisation constant (nonlinearity.py:99),
-which depends on the removed simps and has no reference test vector.
 """
 
 import math
+import os
+
 import numpy as np
 
 __all__ = [
-    "Domain", "Sech", "Gaussian", "Linearity", "Nonlinearity", "Amplifier",
+    "Domain", "Sech", "Gaussian", "Linearity", "Nonlinearity", "Amplifier", "Amplifier2LevelModel",
     "Solver", "Stepper", "Storage", "Fibre", "System",
     "fft", "ifft", "fftshift", "ifftshift", "temporal_power", "spectral_power",
     "energy", "simps_avg", "get_fft_counter", "reset_fft_counter",
@@ -471,6 +477,111 @@ class Amplifier:
 
 
 # --------------------------------------------------------------------------
+# amplifier.py: two-level Yb rate-equation gain
+# --------------------------------------------------------------------------
+YB_CROSS_SECTIONS = os.path.join(os.path.dirname(__file__), "..", "pyofsschanged_amd", "data", "CrossSectionDataExp.txt")
+
+
+def yb_cross_section_table(path=YB_CROSS_SECTIONS):
+    """The table read by amplifier.py:247-258 with pandas.read_csv(..., ",")
+    and its default header=0: the first data row is consumed as column names,
+    so the spline starts from the second row. Columns: wavelength (m),
+    sigma_12 (absorption, m^2), sigma_21 (emission, m^2)."""
+    rows = []
+    with open(path) as fh:
+        lines = [ln for ln in fh.read().splitlines() if ln.strip()]
+    for ln in lines[1:]:
+        rows.append([float(v) for v in ln.split(",")])
+    return np.array(rows)
+
+
+class Amplifier2LevelModel:
+    """amplifier.py:178-402, the CPU path (factor, calculate_N2, g_s, g_p,
+    update_Pp) without the OpenCL helpers."""
+
+    def __init__(self, name="amplifier2LevelModel", Pp=None, N=None, Rr=None):
+        self.h_p = 6.62 * 1e-34
+        self.T = 850.0 * 1e-6
+        self.a = 3.0 * 1e-6
+        self.b = 60.0 * 1e-6
+        self.NA = 0.13
+        self.lamb_p = 976.0
+        self.N = N * (1e25) * np.pi * self.a ** 2
+        self._Pp = Pp
+        self.Tr = 1 / Rr
+        self.domain = None
+        self.sigma12_p = 2.5 * 1e-24
+        self.sigma21_p = 2.44 * 1e-24
+
+    def interpolate_sigma_s(self):
+        """amplifier.py:247-258 (scipy InterpolatedUnivariateSpline, k=3,
+        zero outside the table)."""
+        from scipy import interpolate
+        tab = yb_cross_section_table()
+        IUS = interpolate.InterpolatedUnivariateSpline
+        spl12 = IUS(tab[:, 0] * 1e9, tab[:, 1] * 1e27, ext=1)
+        spl21 = IUS(tab[:, 0] * 1e9, tab[:, 2] * 1e27, ext=1)
+        self.sigma12_s = spl12(self.domain.Lambda) * 1e-27
+        self.sigma21_s = spl21(self.domain.Lambda) * 1e-27
+
+    def set_domain(self, domain):
+        """amplifier.py:260-266"""
+        self.domain = domain
+        self.interpolate_sigma_s()
+        self.shape = domain.nu.shape
+        self.physical_power_factor = np.float64(self.shape[0] * domain.dt / self.Tr)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            V = (2.0 * np.pi / domain.Lambda) * self.a * self.NA * 1e9               # :270-276
+            omega_s = self.a * (0.616 + 1.66 / pow(V, 1.5) + 0.987 / pow(V, 6.0))
+            Gamma_s = 1.0 - np.exp(-2.0 * self.a ** 2 / (omega_s * omega_s))
+            self.rho_s = Gamma_s / (np.pi * self.a ** 2)
+            Gamma_p = (self.a ** 2) / (self.b ** 2)                                 # :278-284
+            self.rho_p = Gamma_p / (np.pi * self.a ** 2)
+            self.Psat_s = (self.h_p * domain.nu * 1e12) / (                          # :286-296
+                self.T * (self.sigma12_s + self.sigma21_s) * self.rho_s)
+            self.Psat_p = (self.h_p * lambda_to_nu(self.lamb_p) * 1e12) / (          # :298-305
+                self.T * (self.sigma12_p + self.sigma21_p) * self.rho_p)
+            self.eta_s = self.sigma12_s * self.rho_s * self.N                        # :307-315
+            self.eta_p = self.sigma12_p * self.rho_p * self.N                        # :317-323
+            self.alpha_s = (self.sigma12_s + self.sigma21_s) * self.rho_s            # :325-333
+            self.alpha_p = (self.sigma12_p + self.sigma21_p) * self.rho_p            # :335-341
+            self.ratio_p = self.sigma12_p / (self.sigma12_p + self.sigma21_p)        # :347-353
+            self.ratio_s = self.sigma12_s / (self.sigma12_s + self.sigma21_s)        # :355-366
+
+    @property
+    def Pp(self):
+        return self._Pp
+
+    def calculate_N2(self, Ps):
+        """amplifier.py:368-375: NaN terms (zero cross sections) are dropped."""
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t1 = self.ratio_s * Ps / self.Psat_s
+            t2 = Ps / self.Psat_s
+        numerator = self.ratio_p * self.Pp / self.Psat_p + np.sum(t1[~np.isnan(t1)])
+        denominator = 1 + self.Pp / self.Psat_p + np.sum(t2[~np.isnan(t2)])
+        return (numerator / denominator) * self.N
+
+    def calculate_g_s(self, N2):
+        return self.alpha_s * N2 - self.eta_s
+
+    def calculate_g_p(self, N2):
+        return self.alpha_p * N2 - self.eta_p
+
+    def update_Pp(self, g_p, h):
+        """amplifier.py:386-388: the pump evolves with every call of factor."""
+        self._Pp = self._Pp * np.exp(g_p * h * 1e3)
+
+    def factor(self, A, h):
+        """amplifier.py:390-402: exponent of the gain of one linear sub-step,
+        from the spectral power of the array passed (one extra FFT)."""
+        N2 = self.calculate_N2(spectral_power(A) * len(A) * self.domain.dt / (self.Tr))
+        g_s = self.calculate_g_s(N2)
+        g_p = self.calculate_g_p(N2)
+        self.update_Pp(g_p, h)
+        return fftshift(g_s * h * 1e3 / 2)
+
+
+# --------------------------------------------------------------------------
 # linearity.py
 # --------------------------------------------------------------------------
 def _factorial(n):
@@ -578,8 +689,8 @@ def calculate_raman_term(domain, tau_1=12.2e-3, tau_2=32.0e-3):
 
 
 def calculate_raman_term_silica(domain):
-    """nonlinearity.py:72-100 (Hollenbeck-Cantrell). Parity unpinned: the
-    normalisation uses the removed simps rule, restated by simps_avg."""
+    """nonlinearity.py:72-100 (Hollenbeck-Cantrell); the normalisation uses
+    the removed simps rule, restated by simps_avg."""
     CP = [56.25, 100.00, 231.25, 362.50, 463.00, 497.00, 611.50, 691.67, 793.67, 835.50, 930.00, 1080.00, 1215.00]
     Ai = [1.00, 11.40, 36.67, 67.67, 74.00, 4.50, 6.80, 4.60, 4.20, 4.50, 2.70, 3.10, 3.00]
     Gfwhm = [52.10, 110.42, 175.00, 162.50, 135.33, 24.50, 41.50, 155.00, 59.50, 64.30, 150.00, 91.00, 160.00]
@@ -1087,13 +1198,13 @@ class Stepper:
 # fibre.py, system.py
 # --------------------------------------------------------------------------
 class Fibre:
-    """fibre.py:35-187 (default sim_type, simple amplifier only)."""
+    """fibre.py:35-187 (default sim_type; simple or two-level Yb amplifier)."""
 
     def __init__(self, name="fibre", length=1.0, alpha=None, beta=None, gamma=0.0, sim_type=None, traces=1,
                  local_error=1.0e-6, method="ss_symmetric", total_steps=100, self_steepening=False,
                  raman_scattering=False, rs_factor=0.003, use_all=False, centre_omega=None, tau_1=12.2e-3,
                  tau_2=32.0e-3, f_R=0.18, small_signal_gain=None, E_sat=None, P_sat=None, Tr=None, lamb0=None,
-                 bandwidth=None, characts=False):
+                 bandwidth=None, use_Yb_model=False, Pp_0=None, N=None, Rr=None, characts=False):
         use_cache = not (method.upper().startswith("A"))
         self.name = name
         self.length = length
@@ -1101,7 +1212,9 @@ class Fibre:
         self.beta_2 = beta[2] if beta is not None else None
         self.domain = None
         self.amplifier = None
-        if (small_signal_gain is not None) and (E_sat is not None or (P_sat is not None and Tr is not None)):
+        if use_Yb_model:  # fibre.py:97-98
+            self.amplifier = Amplifier2LevelModel(Pp=Pp_0, N=N, Rr=Rr)
+        elif (small_signal_gain is not None) and (E_sat is not None or (P_sat is not None and Tr is not None)):
             self.amplifier = Amplifier(gain=small_signal_gain, E_sat=E_sat, P_sat=P_sat, Tr=Tr, length=self.length,
                                        lamb0=lamb0, bandwidth=bandwidth)
         self.linearity = Linearity(alpha=alpha, beta=beta, sim_type=sim_type, use_cache=use_cache,
