@@ -116,6 +116,26 @@ int gnlse_set_nonlinear(gnlse_ctx* ctx, int kind, double gamma, double f_R, doub
 int gnlse_set_gain(gnlse_ctx* ctx, int enabled, double gain_log, double length, double E_sat,
                    const double* profile, const double* simpson_w);
 
+/* Two-level Yb rate-equation gain coupled into every linear sub-step.
+ * Replaces Amplifier2LevelModel.factor (amplifier.py:368-402) as called from
+ * linearity.py:248-272: for the array A passed to a linear call, with
+ * Ps = |fft(A)|^2 * power_factor (power_factor = len(A) dt / Tr),
+ *   N2 = N (ratio_p Pp/Psat_p + sum ratio_s Ps/Psat_s) / (1 + Pp/Psat_p + sum Ps/Psat_s),
+ * the spectrum is multiplied by exp((alpha_s N2 - eta_s) h 1e3 / 2) and the
+ * pump power becomes Pp exp((alpha_p N2 - eta_p) h 1e3) (amplifier.py:386-388).
+ * Arrays float64[n], FFT order: ratio_s/Psat_s, 1/Psat_s, alpha_s, eta_s,
+ * finite (0 where the cross sections vanish: the terms the reference drops as
+ * NaN). gnlse_set_gain(ctx, 0, ...) switches any gain off. */
+int gnlse_set_yb_gain(gnlse_ctx* ctx, const double* ratio_over_psat, const double* inv_psat, const double* alpha_s,
+                      const double* eta_s, double power_factor, double ratio_p, double psat_p, double n_total,
+                      double alpha_p, double eta_p);
+
+/* Pump power Pp per instance (Amplifier2LevelModel._Pp, amplifier.py:201),
+ * read by every launch with the Yb gain (instance b uses pump[b]; count must
+ * cover the batch) and written back at its end. */
+int gnlse_set_pump(gnlse_ctx* ctx, int64_t count, const double* pump);
+int gnlse_get_pump(gnlse_ctx* ctx, int64_t count, double* pump);
+
 /* sign=+1: pyofss field.fft (scipy ifft, 1/N); sign=-1: field.ifft
  * (scipy fft). Replaces field.py:109-136. */
 int gnlse_fft(gnlse_ctx* ctx, int sign, int64_t batch, const double* in, double* out);

@@ -13,7 +13,7 @@ from .domain import (Domain, align_with_nu_grid, dlambda_to_dnu, dnu_to_dlambda,
 from .field import (chirp, energy, fft, fftshift, get_downsampled, get_duration, get_duration_spec, ifft, ifftshift,
                     inst_freq, loss_infrared_db, loss_rayleigh_db, phase, spectral_power, spectrum_width_params,
                     temporal_power)
-from .modules.amplifier import Amplifier
+from .modules.amplifier import Amplifier, Amplifier2LevelModel
 from .modules.fibre import Fibre
 from .modules.gaussian import Gaussian
 from .modules.linearity import (Linearity, convert_alpha_to_dB, convert_alpha_to_linear,

@@ -85,6 +85,9 @@ SIGNATURES = {
     "gnlse_set_linear": [_P, _P],
     "gnlse_set_nonlinear": [_P, _INT, _D, _D, _D, _D, _P, _P, _P],
     "gnlse_set_gain": [_P, _INT, _D, _D, _D, _P, _P],
+    "gnlse_set_yb_gain": [_P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D],
+    "gnlse_set_pump": [_P, _I64, _P],
+    "gnlse_get_pump": [_P, _I64, _P],
     "gnlse_fft": [_P, _INT, _I64, _P, _P],
     "gnlse_apply_linear": [_P, _D, _I64, _P, _P],
     "gnlse_apply_linear_factor": [_P, _I64, _P, _P],
@@ -199,6 +202,9 @@ class Context:
         self.device = device
         self.n = None
         self.lock = threading.RLock()
+        # Object with a pump power `_Pp` (an Amplifier2LevelModel) whose state
+        # every launch with the Yb gain starts from and writes back
+        self.pump_source = None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -237,6 +243,36 @@ class Context:
         self._call("gnlse_set_gain", int(bool(enabled)), float(gain_log), float(length), float(E_sat), _ptr(pr),
                    _ptr(sw))
 
+    def set_yb_gain(self, ratio_over_psat, inv_psat, alpha_s, eta_s, power_factor, ratio_p, psat_p, n_total,
+                    alpha_p, eta_p):
+        """Two-level Yb gain (arrays in FFT order, finite)."""
+        arrs = [_f64(a, self.n) for a in (ratio_over_psat, inv_psat, alpha_s, eta_s)]
+        self._call("gnlse_set_yb_gain", *[_ptr(a) for a in arrs], float(power_factor), float(ratio_p), float(psat_p),
+                   float(n_total), float(alpha_p), float(eta_p))
+
+    def set_pump(self, pump):
+        """Pump power per instance for launches with the Yb gain."""
+        p = np.ascontiguousarray(np.atleast_1d(np.asarray(pump, dtype=np.float64)))
+        self._call("gnlse_set_pump", p.size, _ptr(p))
+
+    def get_pump(self, count):
+        p = np.empty(int(count), dtype=np.float64)
+        self._call("gnlse_get_pump", p.size, _ptr(p))
+        return p
+
+    def _pump_in(self, batch):
+        if self.pump_source is not None:
+            self.set_pump(np.full(int(batch), float(self.pump_source._Pp)))
+
+    def _pump_out(self, batch):
+        """After a launch: the pump powers per instance; a single instance's
+        becomes the amplifier's state, as the reference's object carries it."""
+        if self.pump_source is not None:
+            p = self.get_pump(batch)
+            self.pump_source.pump_out = p
+            if int(batch) == 1:
+                self.pump_source._Pp = float(p[0])
+
     # -- operators ------------------------------------------------------------
     def _batched(self, A):
         A = _c128(A, self.n)
@@ -252,7 +288,9 @@ class Context:
     def apply_linear(self, A, h):
         a, shape = self._batched(A)
         out = np.empty_like(a)
+        self._pump_in(a.shape[0])
         self._call("gnlse_apply_linear", float(h), a.shape[0], _ptr(a), _ptr(out))
+        self._pump_out(a.shape[0])
         return out.reshape(shape)
 
     def apply_linear_factor(self, A):
@@ -281,7 +319,9 @@ class Context:
         (default h/2)."""
         a, shape = self._batched(A)
         out = np.empty_like(a)
+        self._pump_in(a.shape[0])
         self._call("gnlse_step", int(method), float(h), float(h_exp or 0.0), a.shape[0], _ptr(a), _ptr(out))
+        self._pump_out(a.shape[0])
         return out.reshape(shape)
 
     # -- propagation ------------------------------------------------------------
@@ -291,9 +331,11 @@ class Context:
         out = np.empty_like(a)
         traces = np.empty((B, max(max_traces, 0), self.n), np.complex128)
         ntr = np.zeros(B, np.int64)
+        self._pump_in(B)
         self._call("gnlse_propagate_fixed", int(method), B, float(h), float(h_exp or 0.0), int(nsteps), int(trace_every),
                    int(max_traces), _ptr(a), _ptr(out), _ptr(traces) if max_traces > 0 else None,
                    _ptr(ntr) if max_traces > 0 else None)
+        self._pump_out(B)
         return out.reshape(shape), traces, ntr
 
     def propagate_adaptive(self, method, A, params, max_traces=0, log_cap=0, replay=None):
@@ -316,10 +358,12 @@ class Context:
             rlen = len(rh)
         mt = max_traces > 0
         lg = log_cap > 0
+        self._pump_in(B)
         self._call("gnlse_propagate_adaptive", int(method), B, ctypes.byref(params), _ptr(rh), _ptr(ra), int(rlen),
                    _ptr(a), _ptr(out), int(max_traces), _ptr(tA) if mt else None, _ptr(tz) if mt else None,
                    _ptr(sz) if mt else None, _ptr(sh) if mt else None, int(log_cap), _ptr(lh) if lg else None,
                    _ptr(ld) if lg else None, _ptr(la) if lg else None, res)
+        self._pump_out(B)
         results = [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
                     "n_traces": r.n_traces, "status": r.status} for r in res]
         return {"A": out.reshape(shape), "traces": tA, "trace_z": tz, "step_z": sz, "step_h": sh,
@@ -346,7 +390,9 @@ class Context:
 
     def propagate_adaptive_device(self, method, batch, params, d_in, d_out):
         res = (InstanceResult * int(batch))()
+        self._pump_in(batch)
         self._call("gnlse_propagate_adaptive_device", int(method), int(batch), ctypes.byref(params), d_in, d_out, res)
+        self._pump_out(batch)
         return [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
                  "n_traces": r.n_traces, "status": r.status} for r in res]
 

@@ -109,7 +109,8 @@ int gnlse_ctx_destroy(gnlse_ctx* c) {
   DevBuf* bufs[] = {&c->lin,  &c->efix,  &c->nlmul, &c->hr,   &c->omega, &c->gprof, &c->simpw, &c->tw,
                     &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
                     &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr,
-                    &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk};
+                    &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk, &c->yb_rs, &c->yb_ip, &c->yb_as, &c->yb_es,
+                    &c->yb_pp};
   for (DevBuf* b : bufs) release(*b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -136,7 +137,7 @@ int gnlse_set_grid(gnlse_ctx* c, int64_t n) {
     RET_IF(upload(c, c->gprof, zeros.data(), n * sizeof(double)));
     RET_IF(upload(c, c->simpw, zeros.data(), n * sizeof(double)));
     RET_IF(ensure(c->efix, 2 * n * sizeof(double)));
-    c->ops.gain = 0;
+    c->ops.gain = GAIN_NONE;
     refresh_ops(c);
   }
   return GNLSE_OK;
@@ -180,7 +181,7 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
   RET_IF(check_ready(c, false, false));
   HIPCHK(hipSetDevice(c->device));
   if (!enabled) {
-    c->ops.gain = 0;
+    c->ops.gain = GAIN_NONE;
     return GNLSE_OK;
   }
   if (!profile || !simpson_w) return fail(GNLSE_ERR_ARG, "profile and simpson_w are required");
@@ -188,11 +189,58 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
   const size_t n = (size_t)c->n;
   RET_IF(upload(c, c->gprof, profile, n * sizeof(double)));
   RET_IF(upload(c, c->simpw, simpson_w, n * sizeof(double)));
-  c->ops.gain = 1;
+  c->ops.gain = GAIN_SATURABLE;
   c->ops.gain_log = gain_log;
   c->ops.gain_len2 = 2.0 * length;
   c->ops.esat = E_sat;
   refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_set_yb_gain(gnlse_ctx* c, const double* ratio_over_psat, const double* inv_psat, const double* alpha_s,
+                      const double* eta_s, double power_factor, double ratio_p, double psat_p, double n_total,
+                      double alpha_p, double eta_p) {
+  RET_IF(check_ready(c, false, false));
+  if (!ratio_over_psat || !inv_psat || !alpha_s || !eta_s) return fail(GNLSE_ERR_ARG, "null Yb array");
+  if (!(psat_p > 0.0) || !std::isfinite(power_factor) || !std::isfinite(n_total))
+    return fail(GNLSE_ERR_ARG, "invalid Yb constants");
+  const size_t n = (size_t)c->n;
+  for (const double* a : {ratio_over_psat, inv_psat, alpha_s, eta_s})
+    for (size_t i = 0; i < n; ++i)
+      if (!std::isfinite(a[i])) return fail(GNLSE_ERR_ARG, "Yb arrays must be finite (zero where a cross section vanishes)");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(upload(c, c->yb_rs, ratio_over_psat, n * sizeof(double)));
+  RET_IF(upload(c, c->yb_ip, inv_psat, n * sizeof(double)));
+  RET_IF(upload(c, c->yb_as, alpha_s, n * sizeof(double)));
+  RET_IF(upload(c, c->yb_es, eta_s, n * sizeof(double)));
+  c->ops.gain = GAIN_YB;
+  c->ops.yb_pfac = power_factor;
+  c->ops.yb_rp = ratio_p;
+  c->ops.yb_psp = psat_p;
+  c->ops.yb_n = n_total;
+  c->ops.yb_ap = alpha_p;
+  c->ops.yb_ep = eta_p;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_set_pump(gnlse_ctx* c, int64_t count, const double* pump) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  if (count < 1 || !pump) return fail(GNLSE_ERR_ARG, "count must be positive and pump non-null");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(upload(c, c->yb_pp, pump, (size_t)count * sizeof(double)));
+  c->yb_count = count;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_get_pump(gnlse_ctx* c, int64_t count, double* pump) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  if (count < 1 || !pump) return fail(GNLSE_ERR_ARG, "count must be positive and pump non-null");
+  if (count > c->yb_count) return fail(GNLSE_ERR_ARG, "count exceeds the pump powers set");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(pump, c->yb_pp.p, (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }
 
@@ -209,6 +257,7 @@ int gnlse_fft(gnlse_ctx* c, int sign, int64_t batch, const double* in, double* o
 
 int gnlse_apply_linear(gnlse_ctx* c, double h, int64_t batch, const double* in, double* out) {
   RET_IF(check_ready(c, true, false));
+  RET_IF(check_pump(c, batch));
   HIPCHK(hipSetDevice(c->device));
   RET_IF(io_prepare(c, batch, in, c->io_in));
   RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
@@ -256,6 +305,7 @@ int gnlse_apply_exp_nonlinear(gnlse_ctx* c, double h, int64_t batch, const doubl
 
 int gnlse_step(gnlse_ctx* c, int method, double h, double h_exp, int64_t batch, const double* in, double* out) {
   RET_IF(check_ready(c, true, true));
+  RET_IF(check_pump(c, batch));
   if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
   HIPCHK(hipSetDevice(c->device));
   RET_IF(io_prepare(c, batch, in, c->io_in));
@@ -274,6 +324,7 @@ int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, dou
   if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
   if (nsteps < 0 || trace_every < 0 || max_traces < 0) return fail(GNLSE_ERR_ARG, "negative count");
   if (max_traces > 0 && (!traces || !n_traces)) return fail(GNLSE_ERR_ARG, "null trace outputs");
+  RET_IF(check_pump(c, batch));
   HIPCHK(hipSetDevice(c->device));
   const int64_t n = c->n;
   // linearity.py:220-224, 258-262: the cached factor exp(factor*h_exp), where
@@ -333,6 +384,7 @@ static int adaptive_params(int method, const gnlse_adaptive_params* p, AdaptiveP
 
 static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm, const double2* in,
                            double2* out) {
+  RET_IF(check_pump(c, batch));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   return timed_end(c, c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p));
@@ -590,6 +642,8 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
       return fail(GNLSE_ERR_UNSUPPORTED, "ensemble grids are 4096, 8192 or 16384 samples, got " + std::to_string(n));
     if (ctxs[i]->device != ctxs[0]->device) return fail(GNLSE_ERR_ARG, "ensemble groups must share one device");
     if (batch[i] < 1 || batch[i] > (1 << 24)) return fail(GNLSE_ERR_ARG, "group batch out of range");
+    if (ctxs[i]->ops.gain == GAIN_YB)
+      return fail(GNLSE_ERR_UNSUPPORTED, "ensembles do not carry the Yb pump state; propagate such groups directly");
   }
   gnlse_ensemble* e = new gnlse_ensemble();
   e->device = ctxs[0]->device;

@@ -54,6 +54,9 @@ struct gnlse_ctx {
   Ops ops{};
   DevBuf ws, io_in, io_out, io_b, tr_A, tr_z, st_z, st_h, lg_h, lg_d, lg_a, rp_h, rp_a, res, ntr;
   DevBuf ch_w, ch_out, ch_npk, ch_pk;  // trace characteristics
+  DevBuf yb_rs, yb_ip, yb_as, yb_es;   // two-level Yb gain arrays
+  DevBuf yb_pp;                        // pump power per instance
+  int64_t yb_count = 0;                // instances with a pump power set
 };
 
 inline int ensure(DevBuf& b, size_t bytes) {
@@ -109,6 +112,18 @@ inline void refresh_ops(gnlse_ctx* c) {
   c->ops.tw = (const double2*)c->tw.p;
   c->ops.gprof = (const double*)c->gprof.p;
   c->ops.simpw = (const double*)c->simpw.p;
+  c->ops.yb_rs = (const double*)c->yb_rs.p;
+  c->ops.yb_ip = (const double*)c->yb_ip.p;
+  c->ops.yb_as = (const double*)c->yb_as.p;
+  c->ops.yb_es = (const double*)c->yb_es.p;
+  c->ops.yb_pp = (double*)c->yb_pp.p;
+}
+
+// With the Yb gain every instance of a launch carries a pump power.
+inline int check_pump(gnlse_ctx* c, int64_t batch) {
+  if (c->ops.gain == GAIN_YB && c->yb_count < batch)
+    return fail(GNLSE_ERR_STATE, "the Yb gain needs a pump power per instance (gnlse_set_pump with count >= batch)");
+  return GNLSE_OK;
 }
 
 inline int check_ready(gnlse_ctx* c, bool need_lin, bool need_nl) {

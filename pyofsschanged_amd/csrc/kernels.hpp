@@ -62,10 +62,11 @@ struct LaunchTable {
 const LaunchTable* find_table(long long n);
 
 #define COMMA ,
-#define GN_SHARED(N, T, K)                       \
+#define GN_SHARED_G(N, T, K, YB)                 \
   __shared__ double lds[FFT<N, T>::LDS_DOUBLES]; \
   __shared__ double red[(T) / 64 + 2];           \
-  Dev<N, T, K> d(lds, red, ops)
+  Dev<N, T, K, YB> d(lds, red, ops)
+#define GN_SHARED(N, T, K) GN_SHARED_G(N, T, K, false)
 
 template <int N, int T>
 __global__ void __launch_bounds__(T) k_fft(Ops ops, int sign, const double2* __restrict__ in, double2* __restrict__ out) {
@@ -85,17 +86,19 @@ __global__ void __launch_bounds__(T) k_make_exp(Ops ops, double h, double2* __re
   d.make_exp(out, h);
 }
 
-template <int N, int T>
+template <int N, int T, bool YB>
 __global__ void __launch_bounds__(T)
     k_linear(Ops ops, double h, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
-  GN_SHARED(N, T, NL_KERR);
+  GN_SHARED_G(N, T, NL_KERR, YB);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
   d.make_exp(ws + S_EH * N, h);
   double2 x[P];
   d.load(x, in + b * N);
+  d.pump_begin(b);
   d.linear(x, ws + S_EH * N, h);
+  d.pump_end(b);
   d.store(out + b * N, x);
 }
 
@@ -140,10 +143,10 @@ __global__ void __launch_bounds__(T) k_exp_nonlin(Ops ops, double h, const doubl
   d.store(out + b * N, x);
 }
 
-template <int N, int T, int METHOD, int K>
+template <int N, int T, int METHOD, int K, bool YB>
 __global__ void __launch_bounds__(T)
     k_step(Ops ops, double h, double h_exp, const double2* __restrict__ in, double2* __restrict__ out, double2* ws_all) {
-  GN_SHARED(N, T, K);
+  GN_SHARED_G(N, T, K, YB);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
@@ -151,15 +154,17 @@ __global__ void __launch_bounds__(T)
   d.make_exp(ws + S_EH * N, h_exp);
   d.load(x, in + b * N);
   d.store(ws + S_A * N, x);
+  d.pump_begin(b);
   integrator<METHOD>(d, x, ws + S_A * N, ws + S_EH * N, h, ws);
+  d.pump_end(b);
   d.store(out + b * N, x);
 }
 
 // stepper.py:164-222 with the cached exp(factor*h/2) in ops.efix
-template <int N, int T, int METHOD, int K>
+template <int N, int T, int METHOD, int K, bool YB>
 __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const double2* __restrict__ in,
                                              double2* __restrict__ out, double2* ws_all) {
-  GN_SHARED(N, T, K);
+  GN_SHARED_G(N, T, K, YB);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
@@ -167,6 +172,7 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
   double2 x[P];
   d.load(x, in + b * N);
   d.store(sA, x);
+  d.pump_begin(b);
   long long n_tr = 0;
   for (long long i = 0; i < prm.nsteps; ++i) {
     integrator<METHOD>(d, x, sA, ops.efix, prm.h, ws);
@@ -177,6 +183,7 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
     }
   }
   d.store(out + b * N, x);
+  d.pump_end(b);
   if (threadIdx.x == 0 && prm.n_traces) prm.n_traces[b] = n_tr;
 }
 
@@ -186,21 +193,21 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // instance b of a batch whose inputs, outputs and workspaces start at in,
 // out and ws_all. With spec != nullptr the pyofss spectrum fft(A_out) is
 // also written to spec + b*N (input of the ensemble coherence).
-template <int N, int T, int METHOD, int K>
+template <int N, int T, int METHOD, int K, bool YB = false>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
                                               double2* ws_all, double2* __restrict__ spec) {
-  Dev<N, T, K> d(lds, red, ops);
+  Dev<N, T, K, YB> d(lds, red, ops);
   constexpr int P = N / T;
   double2* ws = ws_all + b * NSLOT * N;
   double2* sA = ws + S_A * N;
-  double2* sC = ws + S_C * N;
   double2* sH = ws + S_H * N;
   double2* sEQ = ws + S_EQ * N;
   double2* sEH = ws + S_EH * N;
   double2 x[P];
   d.load(x, in + b * N);
   d.store(sA, x);
+  d.pump_begin(b);
 
   double2* sNA = ws + S_NA * N;
   bool nA_valid = false;  // sNA holds N(sA)
@@ -239,24 +246,26 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       const double h_half = 0.5 * h;
       d.make_exp(sEH, 0.5 * h);
       d.make_exp(sEQ, 0.5 * h_half);
-      // stepper.py:305-308: coarse (h), then half and fine (h/2 twice)
+      // stepper.py:303-308 in its order (the Yb pump state depends on it):
+      // half step, fine step (both h/2), then the coarse step h. The fine
+      // result replaces the half-step field in sH; the coarse one stays in x.
 #pragma unroll 1
       for (int c = 0; c < 3; ++c) {
-        const double2* src = (c == 2) ? sH : sA;
-        const double2* E = (c == 0) ? sEH : sEQ;
-        // N(A) is shared by the coarse and the first half step, and by every
+        const double2* src = (c == 1) ? sH : sA;
+        const double2* E = (c == 2) ? sEH : sEQ;
+        // N(A) is shared by the half and the coarse step, and by every
         // attempt from the same accepted field
-        integrator<METHOD>(d, x, src, E, c == 0 ? h : h_half, ws, sNA, c < 2 && nA_valid, c < 2 && !nA_valid);
-        if (c < 2) nA_valid = true;
-        if (c == 0) d.store(sC, x);
-        else if (c == 1) d.store(sH, x);
+        const bool from_a = (c != 1);
+        integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
+        if (from_a) nA_valid = true;
+        if (c < 2) d.store(sH, x);
       }
       // stepper.py:230-245 relative_local_error
       double sd = 0.0, sf = 0.0, bad = 0.0;
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        const double2 f = x[k];
-        const double2 df = csub(f, sC[d.idx(k)]);
+        const double2 f = sH[d.idx(k)];
+        const double2 df = csub(f, x[k]);
         sd += abs2(df);
         sf += abs2(f);
         if (!(isfinite(f.x) && isfinite(f.y))) bad = 1.0;
@@ -290,7 +299,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         // local extrapolation, stepper.py:336-338
 #pragma unroll
         for (int k = 0; k < P; ++k)
-          x[k] = csub(cscale(x[k], prm.f_alpha), cscale(sC[d.idx(k)], prm.f_beta));
+          x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
         d.store(sA, x);
         nA_valid = false;
         if (z > z_ignore) {  // stepper.py:342-345
@@ -315,6 +324,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
     }
   }
   if (!finished && status == GNLSE_STATUS_OK) status = GNLSE_STATUS_MAX_STEPS;
+  d.pump_end(b);
   d.load(x, sA);
   d.store(out + b * N, x);
   if (spec) {
@@ -334,12 +344,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
 }
 
-template <int N, int T, int METHOD, int K>
+template <int N, int T, int METHOD, int K, bool YB>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
   __shared__ double lds[FFT<N, T>::LDS_DOUBLES];
   __shared__ double red[T / 64 + 2];
-  adaptive_body<N, T, METHOD, K>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
+  adaptive_body<N, T, METHOD, K, YB>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
@@ -350,6 +360,17 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
     if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR>), __VA_ARGS__);       \
     else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL>), __VA_ARGS__);    \
     else hipLaunchKernelGGL((KERNEL, NL_GENERIC>), __VA_ARGS__);                             \
+  } while (0)
+// Propagating kernels also dispatch on the gain model: the Yb gain has its
+// own instantiations (full Raman + self-steepening operator, or runtime kind).
+#define GN_BY_KIND_GAIN(ops, KERNEL, ...)                                                                 \
+  do {                                                                                                  \
+    if ((ops).gain == GAIN_YB) {                                                                        \
+      if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, true>), __VA_ARGS__);            \
+      else hipLaunchKernelGGL((KERNEL, NL_GENERIC, true>), __VA_ARGS__);                                \
+    } else if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR, false>), __VA_ARGS__);    \
+    else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, false>), __VA_ARGS__);        \
+    else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false>), __VA_ARGS__);                                 \
   } while (0)
 
 template <int N, int T>
@@ -364,7 +385,8 @@ struct Launcher {
   }
   static hipError_t linear(hipStream_t s, const Ops& o, double h, long long batch, const double2* in, double2* out,
                            double2* ws) {
-    hipLaunchKernelGGL((k_linear<N, T>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    if (o.gain == GAIN_YB) hipLaunchKernelGGL((k_linear<N, T, true>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    else hipLaunchKernelGGL((k_linear<N, T, false>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t linfac(hipStream_t s, const Ops& o, long long batch, const double2* in, double2* out) {
@@ -383,21 +405,21 @@ struct Launcher {
   }
   static hipError_t step(hipStream_t s, const Ops& o, int method, double h, double h_exp, long long batch, const double2* in,
                          double2* out, double2* ws) {
-    if (method == M_RK4IP) GN_BY_KIND(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
-    else GN_BY_KIND(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    else GN_BY_KIND_GAIN(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t fixed(hipStream_t s, const Ops& o, int method, const FixedParams& p, long long batch,
                           const double2* in, double2* out, double2* ws) {
-    if (method == M_RK4IP) GN_BY_KIND(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
-    else GN_BY_KIND(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t adaptive(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
                              const double2* in, double2* out, double2* ws) {
     if (method == M_RK4IP)
-      GN_BY_KIND(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
-    else GN_BY_KIND(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t characts(hipStream_t s, long long count, const double2* A, const double2* S, const double* w,

@@ -8,6 +8,7 @@
 //   pyofss/modules/linearity.py:241-272    exp_lin (with amplifier coupling)
 //   pyofss/modules/nonlinearity.py:201-270 non / exp_non variants
 //   pyofss/modules/amplifier.py:149-158    saturable gain factor
+//   pyofss/modules/amplifier.py:368-402    two-level Yb gain factor and pump
 //   pyofss/modules/solver.py:266-275, 339-354  ss_symmetric, rk4ip
 // with the pyofss transform convention of pyofss/field.py:109-136:
 //   fft(A)  = (1/N) sum_n A_n exp(+2 pi i k n / N)   = F+(A)/N
@@ -22,7 +23,9 @@ namespace gn {
 enum NlKind { NL_GENERIC = -1, NL_KERR = 0, NL_ALL = 1, NL_SS = 2, NL_RS = 3, NL_SS_RS = 4 };
 enum Method { M_RK4IP = 0, M_SS_SYMMETRIC = 1 };
 // spectral multiplier kinds
-enum MulKind { MUL_ARRAY = 0, MUL_OMEGA = 1, MUL_IOMEGA = 2, MUL_GAIN = 3 };
+enum MulKind { MUL_ARRAY = 0, MUL_OMEGA = 1, MUL_IOMEGA = 2, MUL_GAIN = 3, MUL_YB = 4 };
+// gain models coupled into the linear operator
+enum GainKind { GAIN_NONE = 0, GAIN_SATURABLE = 1, GAIN_YB = 2 };
 
 struct Ops {
   int nl_kind;
@@ -40,6 +43,17 @@ struct Ops {
   double esat;              // saturation energy
   const double* gprof;      // spectral gain profile, FFT order
   const double* simpw;      // Simpson weights of the energy integral
+  // two-level Yb model (amplifier.py:178-402), arrays in FFT order; terms of
+  // zero cross section (NaN or inf in the reference) hold 0
+  const double* yb_rs;      // ratio_s / Psat_s
+  const double* yb_ip;      // 1 / Psat_s
+  const double* yb_as;      // alpha_s
+  const double* yb_es;      // eta_s
+  double yb_pfac;           // len(A) * dt / Tr: spectral power -> watts
+  double yb_rp, yb_psp;     // ratio_p, Psat_p
+  double yb_n;              // total ion number N
+  double yb_ap, yb_ep;      // alpha_p, eta_p
+  double* yb_pp;            // pump power per instance (read at start, written at end)
 };
 
 __device__ __forceinline__ double2 cexp(double2 z) {
@@ -73,7 +87,9 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
 // KIND fixes the nonlinear operator at compile time (NL_KERR, NL_ALL) or
 // leaves it to ops.nl_kind (NL_GENERIC); a fixed kind lets the compiler drop
 // the other operators and their register live ranges.
-template <int N, int T, int KIND>
+// YB compiles the two-level Yb gain into the linear operator (its branch
+// costs registers, so kernels without it do not carry it).
+template <int N, int T, int KIND, bool YB = false>
 struct Dev {
   using F = FFT<N, T>;
   static constexpr int P = N / T;
@@ -83,6 +99,7 @@ struct Dev {
   double* red;
   const Ops& op;
   typename F::Twiddles tw;
+  double pp = 0.0;  // Yb pump power of this instance (identical in every thread)
 
   __device__ Dev(double* l, double* r, const Ops& o) : lds(l), red(r), op(o) { F::load_twiddles(tw, o.tw); }
 
@@ -130,10 +147,32 @@ struct Dev {
         } else if (mul == MUL_IOMEGA) {
 #pragma unroll
           for (int k = 0; k < P; ++k) y[k] = conjg(cmul(make_double2(0.0, op.omega[idx(k)]), cscale(y[k], INVN)));
-        } else {
+        } else if (mul == MUL_GAIN) {
 #pragma unroll
           for (int k = 0; k < P; ++k)
             y[k] = conjg(cscale(cmul(arr[idx(k)], cscale(y[k], INVN)), exp(g * op.gprof[idx(k)])));
+        } else if constexpr (YB) {
+          // amplifier.py:368-402 with g = h: inversion N2 from the spectral
+          // power of the array being propagated (its fft is y), the signal
+          // gain exp(g_s h 1e3 / 2) and the pump update exp(g_p h 1e3)
+          double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            y[k] = cscale(y[k], INVN);
+            const double ps = abs2(y[k]) * op.yb_pfac;
+            s1 += op.yb_rs[idx(k)] * ps;
+            s2 += op.yb_ip[idx(k)] * ps;
+          }
+          s1 = block_sum(s1);
+          s2 = block_sum(s2);
+          const double n2 = (((op.yb_rp * pp) / op.yb_psp + s1) / (1.0 + pp / op.yb_psp + s2)) * op.yb_n;
+          const double hg = g * 1e3;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            const double gs = op.yb_as[idx(k)] * n2 - op.yb_es[idx(k)];
+            y[k] = conjg(cscale(cmul(arr[idx(k)], y[k]), exp(gs * hg * 0.5)));
+          }
+          pp = pp * exp((op.yb_ap * n2 - op.yb_ep) * hg);
         }
       }
     }
@@ -154,10 +193,25 @@ struct Dev {
     return s;
   }
 
-  // linearity.py:241-272 with amplifier.py:149-158:
-  // x <- ifft(exp(G0/(1+E(x)/E_sat) * profile) * E * fft(x))
+  // Pump power of instance b (the Yb model only): read before the first
+  // linear call, written back after the last.
+  __device__ __forceinline__ void pump_begin(size_t b) {
+    if constexpr (YB) pp = op.yb_pp[b];
+  }
+  __device__ __forceinline__ void pump_end(size_t b) const {
+    if constexpr (YB) {
+      if (threadIdx.x == 0) op.yb_pp[b] = pp;
+    }
+  }
+
+  // linearity.py:241-272 with amplifier.py:149-158 (saturable gain):
+  // x <- ifft(exp(G0/(1+E(x)/E_sat) * profile) * E * fft(x)); with the Yb
+  // model (amplifier.py:390-402) the gain comes from the spectrum of x and
+  // each call advances the pump power pp.
   __device__ __forceinline__ void linear(double2 (&x)[P], const double2* __restrict__ E, double h) {
-    if (op.gain) {
+    if constexpr (YB) {
+      spectral(x, MUL_YB, E, h);
+    } else if (op.gain == GAIN_SATURABLE) {
       double e = 0.0;
 #pragma unroll
       for (int k = 0; k < P; ++k) e += op.simpw[idx(k)] * abs2(x[k]);
@@ -340,15 +394,17 @@ struct Dev {
 };
 
 // Per-instance scratch slots (N complex each).
-enum Slot { S_A = 0, S_C, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, S_NA, NSLOT };
+enum Slot { S_A = 0, S_NA, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, NSLOT };
 
-// solver.py:339-354 rk4ip as a 7-stage loop (one call site per operator).
+// solver.py:339-354 rk4ip as a 7-stage loop (one call site per operator),
+// with the linear calls in the reference's order (L(A), L(h N(A)),
+// L(A_I + k2), L(acc)), which the Yb gain's pump state depends on.
 // Input field in slot `in`; E = exp(factor*h/2). Result in x. N(in) is read
 // from nA instead of being computed when nA_ready, and stored to nA when
 // nA_store (the step-doubling controller evaluates N of the same field for
-// its coarse step and its first half step).
-template <int N, int T, int K>
-__device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+// its half step and its coarse step).
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
                       double2* ws, double2* nA = nullptr, bool nA_ready = false, bool nA_store = false) {
   constexpr int P = N / T;
   double2* sAI = ws + S_AI * N;
@@ -360,30 +416,27 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], cons
 #pragma unroll 1
   for (int st = 0; st < 7; ++st) {
     // pre
-    if ((st == 0 && !nA_ready) || st == 2) d.load(x, in);
-    else if (st == 3) {  // A_I + 0.5*k0
-#pragma unroll
-      for (int k = 0; k < P; ++k) x[k] = cadd(x[k], cscale(sK0[d.idx(k)], 0.5));
-    } else if (st == 4) {  // A_I + 0.5*k1
+    if (st == 0 || (st == 1 && !nA_ready)) d.load(x, in);
+    else if (st == 3 || st == 4) {  // A_I + 0.5*k0, A_I + 0.5*k1
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = cadd(sAI[d.idx(k)], cscale(x[k], 0.5));
     } else if (st == 6) d.load(x, sK0);
-    if (st == 1 || st == 2 || st == 5 || st == 6) d.linear(x, E, hh);
-    if (st == 0 || st == 3 || st == 4 || st == 5) {
-      if (st == 0 && nA_ready) {
+    if (st == 0 || st == 2 || st == 5 || st == 6) d.linear(x, E, hh);
+    if (st == 1 || st == 3 || st == 4 || st == 5) {
+      if (st == 1 && nA_ready) {
         d.load(x, nA);
       } else {
         d.nonlin(x, sU, sT);
-        if (st == 0 && nA_store) d.store(nA, x);
+        if (st == 1 && nA_store) d.store(nA, x);
       }
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
     }
     // post
-    if (st == 1) d.store(sK0, x);        // k0 = L(h N(A))
-    else if (st == 2) d.store(sAI, x);   // A_I = L(A)
-    else if (st == 3) d.store(sK1, x);   // k1
-    else if (st == 4) {                  // k2 in x
+    if (st == 0) d.store(sAI, x);         // A_I = L(A)
+    else if (st == 2) d.store(sK0, x);    // k0 = L(h N(A))
+    else if (st == 3) d.store(sK1, x);    // k1
+    else if (st == 4) {                   // k2 in x
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         const int i = d.idx(k);
@@ -405,8 +458,8 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K>& d, double2 (&x)[N / T], cons
 
 // solver.py:266-275 ss_symmetric: L(h/2), exp_non with the phase of the
 // un-propagated input, L(h/2).
-template <int N, int T, int K>
-__device__ __forceinline__ void ss_symmetric(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void ss_symmetric(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
                              double2* ws) {
   double2* sT = ws + S_T * N;
   double2* sU = ws + S_U * N;
@@ -419,8 +472,8 @@ __device__ __forceinline__ void ss_symmetric(Dev<N, T, K>& d, double2 (&x)[N / T
   }
 }
 
-template <int METHOD, int N, int T, int K>
-__device__ __forceinline__ void integrator(Dev<N, T, K>& d, double2 (&x)[N / T], const double2* in, const double2* E,
+template <int METHOD, int N, int T, int K, bool YB>
+__device__ __forceinline__ void integrator(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E,
                                            double h, double2* ws, double2* nA = nullptr, bool nA_ready = false,
                                            bool nA_store = false) {
   if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);

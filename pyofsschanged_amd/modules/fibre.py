@@ -10,7 +10,7 @@ from numpy import abs, amax
 
 from .. import _lib, field
 from ..field import get_duration, temporal_power
-from .amplifier import Amplifier
+from .amplifier import Amplifier, Amplifier2LevelModel
 from .linearity import Linearity
 from .nonlinearity import Nonlinearity, _NL_FFTS
 from .stepper import Stepper
@@ -41,9 +41,9 @@ class Fibre:
         self.cycle = cycle
         self.domain = None
         self.amplifier = None
-        if use_Yb_model:
-            raise NotImplementedError("the two-level Yb gain model is not supported by the HIP engine")
-        if (small_signal_gain is not None) and (E_sat is not None or (P_sat is not None and Tr is not None)):
+        if use_Yb_model:  # fibre.py:97-98
+            self.amplifier = Amplifier2LevelModel(Pp=Pp_0, N=N, Rr=Rr)
+        elif (small_signal_gain is not None) and (E_sat is not None or (P_sat is not None and Tr is not None)):
             self.amplifier = Amplifier(gain=self.small_signal_gain, E_sat=E_sat, P_sat=P_sat, Tr=Tr,
                                        length=self.length, lamb0=lamb0, bandwidth=bandwidth,
                                        use_Er_profile=use_Er_profile)
@@ -102,7 +102,8 @@ class Fibre:
     def _count_ffts(self, method, calls):
         """Advance field.fft_counter as pyofss counts FFTs (field.py:118-134)."""
         nl = _NL_FFTS[self.nonlinearity.kind]
-        per_call = 8 + 4 * nl if method == _lib.METHOD_RK4IP else 4 + nl
+        g = self.linearity.gain_ffts()  # per linear call: 4 in rk4ip, 2 in ss_symmetric
+        per_call = 8 + 4 * nl + 4 * g if method == _lib.METHOD_RK4IP else 4 + nl + 2 * g
         field.fft_counter += per_call * calls
 
     def prepare(self, domain):

@@ -110,10 +110,23 @@ class Linearity:
     def _upload(self):
         ctx = self._context()
         ctx.set_linear(np.broadcast_to(np.asarray(self.factor, dtype=np.complex128), (self.domain.total_samples,)))
-        if self.amplifier is not None:
+        ctx.pump_source = None
+        if self._yb():
+            ctx.set_yb_gain(*self.amplifier.device_arrays())
+            ctx.pump_source = self.amplifier  # its Pp travels with every launch
+        elif self.amplifier is not None:
             ctx.set_gain(True, *self.amplifier.device_arrays())
         else:
             ctx.set_gain(False)
+
+    def _yb(self):
+        from .amplifier import Amplifier2LevelModel
+        return isinstance(self.amplifier, Amplifier2LevelModel)
+
+    def gain_ffts(self):
+        """FFTs pyofss counts inside the amplifier per exp_lin call: the Yb
+        model's spectral_power (amplifier.py:397)."""
+        return 1 if self._yb() else 0
 
     def default_cache(self, h):
         """linearity.py:220-224: fix the step size of the cached factor."""
@@ -129,12 +142,12 @@ class Linearity:
 
     def default_exp_f(self, A, h):
         """linearity.py:241-256: ifft(exp(factor*h) [* exp(gain)] * fft(A)) on the GPU."""
-        field.fft_counter += 2
+        field.fft_counter += 2 + self.gain_ffts()
         return self._context().apply_linear(A, h)
 
     def default_exp_f_cached(self, A, h):
         """linearity.py:258-272: as default_exp_f with the step of the first call."""
         if self.cached_factor is None:
             self.cache(h)
-        field.fft_counter += 2
+        field.fft_counter += 2 + self.gain_ffts()
         return self._context().apply_linear(A, self.cached_factor)

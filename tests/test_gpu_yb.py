@@ -1,0 +1,173 @@
+"""GPU parity of the two-level Yb gain (Amplifier2LevelModel,
+pyofss/modules/amplifier.py:178-469, coupled at linearity.py:248-272).
+
+* Operator: a sequence of Linearity.exp_lin calls (the pump power carries
+  from call to call) against the oracle: fields <= 1e-12 relative L2, pump
+  power <= 1e-12 relative.
+* Fixed-step rk4ip and ss_symmetric, adaptive ark4ip: device propagation
+  against the oracle on the same inputs: traces <= 1e-10, final pump power
+  <= 1e-10, identical attempt counts for the adaptive run.
+* The published run of examples/notebooks/nonlinear_attractor.ipynb (2^14
+  samples, 30000 rk4ip steps over 8 m, Hollenbeck Raman, Yb gain, 500
+  traces), written exactly as the notebook writes it with this package's
+  System and Fibre: df_results against the notebook's tables
+  (tests/golden/yb_attractor.json). Peak power and energy of the first five
+  traces within 1e-11 of the OpenCL column's full digits (the oracle itself
+  is within 2.4e-13). Last five traces: peak power within 5e-9 of the OpenCL
+  column (the reference's CPU and OpenCL runs differ by 6.7e-10 there); peak
+  power, energy and the final pump power within 1e-9 of the oracle's
+  full-length run (tests/golden/yb_attractor_oracle.json, made by
+  make_yb_attractor_oracle.py). The OpenCL column's energies of the last
+  traces are not used: they differ by ~5e-9 from both the oracle and this
+  engine, which agree to ~1e-11 (the OpenCL path integrates the energy with
+  its own kernel). All columns against the CPU column's six printed decimals.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "yb_attractor.json")))
+ORACLE_RUN = json.load(open(os.path.join(HERE, "golden", "yb_attractor_oracle.json")))
+CFG = GOLD["config"]
+
+
+@pytest.fixture(scope="module")
+def P():
+    import pyofsschanged_amd as P
+    from pyofsschanged_amd import _lib
+    if _lib.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return P
+
+
+def domains(P, samples=None):
+    n = samples or CFG["samples"]
+    od = O.Domain(samples_per_bit=n, bit_width=CFG["bit_width_ps"], total_bits=1,
+                  centre_nu=O.lambda_to_nu(CFG["lambda0_nm"]))
+    pd_ = P.Domain(samples_per_bit=n, bit_width=CFG["bit_width_ps"], total_bits=1,
+                   centre_nu=P.lambda_to_nu(CFG["lambda0_nm"]))
+    A = O.Gaussian(peak_power=CFG["peak_power_W"], width=CFG["fwhm_ps"], C=0, using_fwhm=True).generate(od.t)
+    return od, pd_, A
+
+
+def fibres(P, length, method, steps, traces, local_error=1e-6, **kw):
+    args = dict(length=length, method=method, total_steps=steps, beta=np.array(CFG["beta"]), gamma=CFG["gamma"],
+                self_steepening=False, use_all="hollenbeck", traces=traces, Pp_0=CFG["Pp_0"], N=CFG["N"],
+                Rr=CFG["Rr"], use_Yb_model=True, local_error=local_error, **kw)
+    return O.Fibre(**args), P.Fibre(**args)
+
+
+def test_linear_operator_sequence(P):
+    od, pd_, A = domains(P)
+    amp_o = O.Amplifier2LevelModel(Pp=10.0, N=4.8, Rr=80e-6)
+    lin_o = O.Linearity(beta=CFG["beta"], amplifier=amp_o)
+    lin_o(od)
+    amp_p = P.Amplifier2LevelModel(Pp=10.0, N=4.8, Rr=80e-6)
+    lin_p = P.Linearity(beta=CFG["beta"], amplifier=amp_p)
+    lin_p(pd_)
+    a_o = A.copy()
+    a_p = A.copy()
+    for h in (1e-6, 3e-7, 2e-5, 1e-6):
+        a_o = lin_o.exp_lin(a_o, h)
+        a_p = lin_p.exp_lin(a_p, h)
+        e = rel_l2(a_p, a_o)
+        print(f"exp_lin h={h}: rel L2 {e:.2e}, Pp {amp_p.Pp:.15g} (oracle {amp_o.Pp:.15g})", flush=True)
+        assert e <= 1e-12
+        assert amp_p.Pp == pytest.approx(amp_o.Pp, rel=1e-12)
+    assert amp_o.Pp != 10.0
+
+
+@pytest.mark.parametrize("method,steps,traces", [("rk4ip", 241, 4), ("ss_symmetric", 121, 4)])
+def test_fixed_steps_match_oracle(P, method, steps, traces):
+    od, pd_, A = domains(P)
+    h0 = CFG["length_km"] / CFG["total_steps"]
+    fo, fp = fibres(P, steps * h0, method, steps, traces)
+    out_o = fo(od, A.copy())
+    out_p = fp(pd_, A.copy())
+    so, sp = fo.stepper.storage, fp.stepper.storage
+    assert len(sp.As) == len(so.As) == 5
+    for k in range(5):
+        e = rel_l2(sp.As[k], so.As[k])
+        print(f"{method} trace {k} z={sp.z[k]:.6e}: rel L2 {e:.2e}", flush=True)
+        assert e <= 1e-10
+    assert rel_l2(out_p, out_o) <= 1e-10
+    print(f"{method}: Pp {fp.amplifier.Pp:.15g} (oracle {fo.amplifier.Pp:.15g})", flush=True)
+    assert fp.amplifier.Pp == pytest.approx(fo.amplifier.Pp, rel=1e-10)
+    assert sp.fft_total == so.fft_total
+
+
+def test_adaptive_matches_oracle(P):
+    """ark4ip: step doubling calls the linear operator 12 times per attempt,
+    in the reference's order (half step, fine step, coarse step), and every
+    call advances the pump, rejected attempts included."""
+    od, pd_, A = domains(P, samples=4096)
+    fo, fp = fibres(P, 2e-6, "ark4ip", 10, 2, local_error=1e-8)
+    out_o = fo(od, A.copy())
+    out_p = fp(pd_, A.copy())
+    e = rel_l2(out_p, out_o)
+    n_o = len(fo.stepper.attempt_log)
+    print(f"ark4ip: rel L2 {e:.2e}, attempts {fp.stepper.attempts} (oracle {n_o}, "
+          f"{sum(1 for a in fo.stepper.attempt_log if not a[2])} rejected), "
+          f"Pp {fp.amplifier.Pp:.15g} (oracle {fo.amplifier.Pp:.15g})", flush=True)
+    assert fp.stepper.attempts == n_o
+    assert e <= 1e-10
+    assert fp.amplifier.Pp == pytest.approx(fo.amplifier.Pp, rel=1e-10)
+
+
+def gold(col, which):
+    rows = sorted(GOLD[which][col].items(), key=lambda kv: float(kv[0]))
+    return [(float(z), v) for z, v in rows]
+
+
+def test_published_attractor_run(P):
+    """examples/notebooks/nonlinear_attractor.ipynb cell 16, first system."""
+    domain = P.Domain(samples_per_bit=2 ** 14, bit_width=200.0, total_bits=1, centre_nu=P.lambda_to_nu(1028))
+    gaussian = P.Gaussian(peak_power=600, width=2, C=0, using_fwhm=True)
+    sys0 = P.System(domain)
+    sys0.add(gaussian)
+    sys0.add(P.Fibre(length=8 * 1e-3, method="rk4ip", total_steps=30000, beta=np.array([0, 0, 20]), gamma=5.1,
+                     self_steepening=False, use_all="hollenbeck", cycle="cycle0", traces=500, Pp_0=10, N=4.8,
+                     Rr=80 * 1e-6, use_Yb_model=True, downsampling=4000))
+    sys0.run()
+    fib = sys0["fibre"]
+    print(f"final pump {fib.amplifier.Pp:.15g} W (oracle {ORACLE_RUN['final_pump_W']:.15g})", flush=True)
+    assert fib.amplifier.Pp == pytest.approx(ORACLE_RUN["final_pump_W"], rel=1e-9)
+    assert fib.stepper.storage.fft_total == ORACLE_RUN["fft_total"]
+    df = sys0.df_results
+    assert len(df) == 500
+    z = df.index.get_level_values("z [mm]").values
+    rows = list(range(5)) + list(range(495, 500))
+    for j, r in enumerate(rows):
+        zg = gold("Peak Power [W]", "opencl")[j][0]
+        assert z[r] == pytest.approx(zg, abs=1e-6)
+        rec = df.iloc[r]
+        bar = 1e-11 if j < 5 else 5e-9
+        p_cl, e_cl = float(gold("Peak Power [W]", "opencl")[j][1]), float(gold("Energy [nJ]", "opencl")[j][1])
+        print(f"z={z[r]:.6f} mm: peak {rec['Peak Power [W]']:.13g} (OpenCL {p_cl:.13g}), energy "
+              f"{rec['Energy [nJ]']:.13g} (OpenCL {e_cl:.13g}), duration {rec['Temp width [ps]']:.6f}, "
+              f"spectral width {rec['Spec width [THz]']:.6f}, peaks {list(rec['Peaks [idx]'])}", flush=True)
+        orc = ORACLE_RUN["rows"][j]
+        assert orc["z_mm"] == pytest.approx(zg, abs=1e-6)
+        assert rec["Peak Power [W]"] == pytest.approx(p_cl, rel=bar)
+        if j < 5:
+            assert rec["Energy [nJ]"] == pytest.approx(e_cl, rel=bar)
+        else:
+            assert rec["Peak Power [W]"] == pytest.approx(orc["peak_W"], rel=1e-9)
+            assert rec["Energy [nJ]"] == pytest.approx(orc["energy_nJ"], rel=1e-9)
+        for col, quant in (("Peak Power [W]", 0), ("Energy [nJ]", 0), ("Temp width [ps]", 0),
+                           ("Dispersion length [km]", 0), ("Nonlinear length [km]", 0),
+                           ("Spec width [THz]", domain.dnu)):
+            cpu = float(gold(col, "cpu")[j][1])
+            assert abs(rec[col] - cpu) <= 6e-7 + 5e-9 * abs(cpu) + quant, col
+        assert [int(i) for i in rec["Peaks [idx]"]] == json.loads(gold("Peaks [idx]", "cpu")[j][1])
+    # the published run's trace frames: 4000-point down-sampled powers
+    assert sys0.df_temp.shape == (500, 4000) and sys0.df_spec.shape == (500, 4000)
