@@ -17,14 +17,17 @@ NumPy oracle on one host core, timed on a bounded sample of the same
 workload). Further objects report the other workloads of the engine:
 "ensemble" (512 noise-seeded soliton realisations on 4096/8192/16384-sample
 grids in one graph-replayed launch with |g12| on the device), "amplifier"
-(64 chirped pulses through the 5 m saturable-gain amplifier, 16384 samples)
-and
-"surrogate" (the spectral neural operator trained here on solver-generated
-data, its throughput against the FP64 solver on the same fields, and its
-spectral error after the judge re-solves gated and audited fields).
+(64 chirped pulses through the 5 m saturable-gain amplifier, 16384 samples),
+"yb_attractor" (the two-level Yb amplifier run whose timing pyofss publishes
+in examples/notebooks/nonlinear_attractor.ipynb, once as the notebook runs it
+and batched over pump powers) and "surrogate" (the spectral neural operator
+trained here on solver-generated data, its throughput against the FP64
+solver on the same fields, and its spectral error after the judge re-solves
+gated and audited fields).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
-                    [--no-ensemble] [--no-surrogate] [--surrogate-steps S]
+                    [--no-ensemble] [--no-amplifier] [--no-yb] [--no-surrogate]
+                    [--surrogate-steps S]
 """
 
 import argparse
@@ -54,6 +57,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the noise-ensemble measurement")
     ap.add_argument("--no-amplifier", action="store_true", help="skip the amplifier-batch measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
+    ap.add_argument("--no-yb", action="store_true", help="skip the Yb attractor measurement")
     ap.add_argument("--surrogate-steps", type=int, default=60000, help="training steps of the surrogate")
     ap.add_argument("--surrogate-train", type=int, default=20000, help="solver runs in the training set")
     ap.add_argument("--surrogate-length", type=float, default=1e-3, help="surrogate fibre segment (km)")
@@ -174,6 +178,49 @@ def amplifier_measurement():
                         "Lorentzian 40 nm gain, adaptive RK4IP",
             "batch": B, "launch_ms": ms, "propagations_per_s": B / wall, "mean_attempts": att,
             "algorithmic_GBps": achieved, "hbm_frac": achieved / HBM_PEAK_GBS}
+
+
+def yb_attractor_measurement(batch):
+    """The Yb amplifier run of examples/notebooks/nonlinear_attractor.ipynb,
+    whose timing is the reference's only published GPU figure: once through
+    System/Fibre exactly as the notebook runs it (wall time of System.run,
+    500 traces and their characteristics included), then `batch` instances
+    with pump powers spread over 5-15 W in one fixed-step launch (output
+    fields only)."""
+    from pyofsschanged_amd import System, workloads
+    from pyofsschanged_amd._lib import METHOD_RK4IP
+    domain, pulse, fibre = workloads.yb_attractor()
+    sys0 = System(domain)
+    sys0.add(pulse)
+    sys0.add(fibre)
+    t0 = time.perf_counter()
+    sys0.run()
+    single = time.perf_counter() - t0
+    peak = float(sys0.df_results["Peak Power [W]"].values[-1])
+    # batched: the same fibre, one instance per pump power
+    ctx = fibre.prepare(domain)
+    A0 = pulse.generate(domain.t)
+    fields = np.ascontiguousarray(np.broadcast_to(A0, (batch, domain.total_samples)))
+    src, ctx.pump_source = ctx.pump_source, None
+    pumps = np.linspace(5.0, 15.0, batch)
+    ctx.set_pump(pumps)
+    h = fibre.length / fibre.stepper.total_steps
+    t0 = time.perf_counter()
+    ctx.propagate_fixed(METHOD_RK4IP, fields, h, fibre.stepper.total_steps, h_exp=fibre.linear_step(0.5 * h))
+    wall = time.perf_counter() - t0
+    ms = ctx.last_kernel_ms()
+    out_pump = ctx.get_pump(batch)
+    ctx.pump_source = src
+    print(f"yb attractor: one run {single:.1f} s, {batch} pump powers in {wall:.1f} s", file=sys.stderr, flush=True)
+    return {"workload": "Yb two-level amplifier of nonlinear_attractor.ipynb: 16384 samples / 200 ps at 1028 nm, "
+                        "600 W 2 ps Gaussian, 8 m, beta2 20 ps^2/km, gamma 5.1 /W/km, Hollenbeck Raman, Yb gain "
+                        "(N 4.8, 80 MHz), 30000 rk4ip steps",
+            "single_run_s": single, "single_run_traces": 500, "final_peak_W": peak,
+            "published_cpu_s": 143.37, "published_opencl_rtx4090_s": 115.50,
+            "speedup_vs_published_opencl": 115.50 / single,
+            "batch": batch, "batch_pump_W": [float(pumps[0]), float(pumps[-1])], "batch_launch_ms": ms,
+            "batch_wall_s": wall, "batch_propagations_per_s": batch / wall,
+            "batch_final_pump_W": [float(out_pump.min()), float(out_pump.max())]}
 
 
 def surrogate_measurement(train_count, steps, length):
@@ -325,6 +372,8 @@ def main():
         line["ensemble"] = ensemble_measurement()
     if not args.no_amplifier:
         line["amplifier"] = amplifier_measurement()
+    if not args.no_yb:
+        line["yb_attractor"] = yb_attractor_measurement(cu_count())
     if not args.no_surrogate:
         line["surrogate"] = surrogate_measurement(args.surrogate_train, args.surrogate_steps, args.surrogate_length)
     if not args.no_cpu:

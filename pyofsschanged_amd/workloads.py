@@ -14,6 +14,12 @@
                 gamma = 5e-3 /W/m, Raman (use_all), 15 dB small-signal gain,
                 E_sat = 5 nJ, Lorentzian gain profile 40 nm wide, 16384
                 samples over 40 ps, adaptive RK4IP.
+  yb_attractor  the two-level Yb amplifier run of pyofss's
+                examples/notebooks/nonlinear_attractor.ipynb: 600 W, 2 ps
+                FWHM Gaussian at 1028 nm, 16384 samples over 200 ps, 8 m of
+                fibre (beta2 = 20 ps^2/km, gamma = 5.1 /W/km, Hollenbeck
+                Raman), Yb gain with 10 W pump, N = 4.8, 80 MHz repetition,
+                30000 fixed rk4ip steps, 500 traces.
   noise_ensemble  realisations of the soliton fibre (adaptive RK4IP, local
                 error 1e-6) on several grids over the same 100 ps window
                 (default 128 x 16384, 256 x 8192, 128 x 4096 samples), each
@@ -67,6 +73,15 @@ def supercontinuum(samples=8192, length=1.5e-4, local_error=1e-6, traces=1):
     pulse = Sech(peak_power=1e4, width=0.050, using_fwhm=True)
     fibre = Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
                   method="ark4ip", local_error=local_error, traces=traces)
+    return domain, pulse, fibre
+
+
+def yb_attractor(traces=500, total_steps=30000, length=8e-3, cycle="cycle0"):
+    domain = Domain(samples_per_bit=2 ** 14, bit_width=200.0, total_bits=1, centre_nu=C_NM_PS / 1028.0)
+    pulse = Gaussian(peak_power=600, width=2, C=0, using_fwhm=True)
+    fibre = Fibre(length=length, method="rk4ip", total_steps=total_steps, beta=np.array([0, 0, 20]), gamma=5.1,
+                  self_steepening=False, use_all="hollenbeck", cycle=cycle, traces=traces, Pp_0=10, N=4.8,
+                  Rr=80 * 1e-6, use_Yb_model=True, downsampling=4000)
     return domain, pulse, fibre
 
 
