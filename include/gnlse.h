@@ -49,6 +49,12 @@ enum {
   GNLSE_NL_SS_RS = 4  /* default_f_with_ss_and_rs / exp variant    :250-264 */
 };
 
+/* down-sampling rules of gnlse_trace_powers */
+enum {
+  GNLSE_DOWNSAMPLE_PYOFSS = 0, /* field.get_downsampled, field.py:300-303 */
+  GNLSE_DOWNSAMPLE_OPENCL = 1  /* cl_interpolate, opencl_fibre.py:207-216 */
+};
+
 /* per-instance termination status; map to pyofss/modules/stepper.py:33-46 */
 enum {
   GNLSE_STATUS_OK = 0,
@@ -204,6 +210,16 @@ int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, c
  * field.py:272-274; at most max_peaks stored, n_peaks counts all). */
 int gnlse_trace_characteristics(gnlse_ctx* ctx, int64_t count, const double* A, const double* simpson_w, double* out,
                                 int32_t* n_peaks, int32_t* peaks, int max_peaks);
+
+/* Down-sampled temporal and spectral powers of `count` fields A (host,
+ * complex128, time domain), n_out points each, into temporal[count][n_out]
+ * and spectral[count][n_out] (spectral power in natural order, as
+ * field.spectral_power). mode GNLSE_DOWNSAMPLE_PYOFSS: get_downsampled of
+ * temporal_power / spectral_power as Storage.get_plot_data does
+ * (storage.py:304-350); GNLSE_DOWNSAMPLE_OPENCL: cl_power, cl_fftshift and
+ * cl_interpolate as OpenclFibre stores its traces (opencl_fibre.py:650-662). */
+int gnlse_trace_powers(gnlse_ctx* ctx, int64_t count, const double* A, int64_t n_out, int mode, double* temporal,
+                       double* spectral);
 
 /* Duration in ms of the last propagation kernel, from HIP events recorded on
  * the context's stream around the launch. */

@@ -19,6 +19,7 @@ from .modules.gaussian import Gaussian
 from .modules.linearity import (Linearity, convert_alpha_to_dB, convert_alpha_to_linear,
                                 convert_dispersion_to_engineering, convert_dispersion_to_physical)
 from .modules.nonlinearity import Nonlinearity, calculate_gamma, calculate_raman_term, calculate_raman_term_silica
+from .modules.opencl_fibre import OpenclFibre, OpenclProgramm
 from .modules.sech import Sech
 from .modules.solver import Solver
 from .modules.stepper import (MaximumStepsAllocatedError, SmallStepSizeError, Stepper, SuitableStepSizeError)

@@ -24,6 +24,9 @@ NL_SS = 2
 NL_RS = 3
 NL_SS_RS = 4
 
+DOWNSAMPLE_PYOFSS = 0
+DOWNSAMPLE_OPENCL = 1
+
 STATUS_OK = 0
 STATUS_SMALL_STEP = 1
 STATUS_SUITABLE_STEP = 2
@@ -105,6 +108,7 @@ SIGNATURES = {
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
+    "gnlse_trace_powers": [_P, _I64, _P, _I64, _INT, _P, _P],
     "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
     "gnlse_ensemble_tune": [_P, _INT, _INT, ctypes.POINTER(ctypes.c_int)],
     "gnlse_ensemble_threads": [_P, ctypes.POINTER(ctypes.c_int)],
@@ -416,6 +420,16 @@ class Context:
             raise GNLSEError(f"more than {max_peaks} temporal peaks in a trace")
         return {"energy": out[:, 0], "max_power": out[:, 1], "duration": out[:, 2], "spec_width": out[:, 3],
                 "peaks": [pk[i, :npk[i]].astype(np.int64) for i in range(k)]}
+
+    def trace_powers(self, A, n_out, mode=DOWNSAMPLE_PYOFSS):
+        """Down-sampled temporal and spectral powers (natural order) of every
+        row of A on the device: (temporal[k, n_out], spectral[k, n_out])."""
+        a, _ = self._batched(A)
+        k = a.shape[0]
+        t = np.empty((k, int(n_out)), dtype=np.float64)
+        sp = np.empty((k, int(n_out)), dtype=np.float64)
+        self._call("gnlse_trace_powers", k, _ptr(a), int(n_out), int(mode), _ptr(t), _ptr(sp))
+        return t, sp
 
 
 def _results(res):

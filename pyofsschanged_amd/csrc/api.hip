@@ -64,6 +64,53 @@ static int io_finish(gnlse_ctx* c, int64_t batch, double* out) {
 
 static bool method_ok(int m) { return m == GNLSE_METHOD_RK4IP || m == GNLSE_METHOD_SS_SYMMETRIC; }
 
+// Down-sampled powers of traces (one workgroup row per trace, a thread per
+// output point). A: fields; S: their pyofss spectra fft(A) in FFT order.
+// PYOFSS: field.get_downsampled (field.py:300-303), i.e. scipy's interp1d
+//   at x_i = i (N-1)/(n_out-1) (numpy linspace), lo = clip(searchsorted(x),
+//   1, N-1) - 1, y = (y[lo+1] - y[lo]) (x - lo) + y[lo], of temporal_power
+//   and of spectral_power (natural order), storage.py:304-350.
+// OPENCL: cl_power, cl_fftshift and cl_interpolate of opencl_fibre.py:199-216,
+//   272-281: t = i/n_out in single precision, j = (int)(t (N-1)),
+//   y = mix(y[j], y[j+1], t (N-1) - j).
+// The products and sums are rounded separately (no contraction), as in numpy.
+constexpr int kPowT = 256;
+
+__device__ __forceinline__ double pow_at(const double2* a, long long k) {
+  const double2 v = a[k];
+  return __dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y));
+}
+
+__global__ void __launch_bounds__(kPowT) k_trace_powers(const double2* __restrict__ A, const double2* __restrict__ S,
+                                                        long long n, long long n_out, int mode,
+                                                        double* __restrict__ out_t, double* __restrict__ out_s) {
+  const long long b = blockIdx.x;
+  const long long i = (long long)blockIdx.y * kPowT + threadIdx.x;
+  if (i >= n_out) return;
+  const double2* a = A + b * n;
+  const double2* s = S + b * n;
+  const long long half = n / 2;
+  long long lo;
+  double u;
+  if (mode == GNLSE_DOWNSAMPLE_PYOFSS) {
+    const double x = (i == n_out - 1) ? (double)(n - 1) : __dmul_rn((double)i, (double)(n - 1) / (double)(n_out - 1));
+    long long hi = (long long)ceil(x);
+    hi = hi < 1 ? 1 : (hi > n - 1 ? n - 1 : hi);
+    lo = hi - 1;
+    u = __dsub_rn(x, (double)lo);
+  } else {
+    const float t = (float)i / (float)n_out;
+    const double tt = __dmul_rn((double)t, (double)(n - 1));
+    lo = (long long)tt;
+    u = __dsub_rn(tt, (double)lo);
+  }
+  const double t0 = pow_at(a, lo), t1 = pow_at(a, lo + 1);
+  // natural (shifted) order: index k holds the FFT-order bin (k + N/2) mod N
+  const double s0 = pow_at(s, (lo + half) % n), s1 = pow_at(s, (lo + 1 + half) % n);
+  out_t[b * n_out + i] = __dadd_rn(__dmul_rn(__dsub_rn(t1, t0), u), t0);
+  out_s[b * n_out + i] = __dadd_rn(__dmul_rn(__dsub_rn(s1, s0), u), s0);
+}
+
 extern "C" {
 
 const char* gnlse_last_error(void) { return g_err.c_str(); }
@@ -533,6 +580,34 @@ int gnlse_trace_characteristics(gnlse_ctx* c, int64_t count, const double* A, co
   if (max_peaks > 0)
     HIPCHK(hipMemcpyAsync(peaks, c->ch_pk.p, (size_t)count * max_peaks * sizeof(int), hipMemcpyDeviceToHost,
                           c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_trace_powers(gnlse_ctx* c, int64_t count, const double* A, int64_t n_out, int mode, double* temporal,
+                       double* spectral) {
+  RET_IF(check_ready(c, false, false));
+  if (count < 1 || count > 0x7fffffff || !A || !temporal || !spectral || n_out < 2 || n_out > (1 << 24) ||
+      (mode != GNLSE_DOWNSAMPLE_PYOFSS && mode != GNLSE_DOWNSAMPLE_OPENCL))
+    return fail(GNLSE_ERR_ARG, "invalid trace power arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t fb = (size_t)count * c->n * sizeof(double2);
+  const size_t ob = (size_t)count * n_out * sizeof(double);
+  RET_IF(io_prepare(c, count, A, c->io_in));
+  RET_IF(ensure(c->io_b, fb));
+  RET_IF(ensure(c->ch_out, 2 * ob));
+  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p);
+  if (e == hipSuccess) {
+    const dim3 grid((unsigned)count, (unsigned)((n_out + kPowT - 1) / kPowT));
+    double* out = (double*)c->ch_out.p;
+    hipLaunchKernelGGL(k_trace_powers, grid, dim3(kPowT), 0, c->stream, (const double2*)c->io_in.p,
+                       (const double2*)c->io_b.p, (long long)c->n, (long long)n_out, mode, out,
+                       out + (size_t)count * n_out);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("trace powers: ") + hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(temporal, c->ch_out.p, ob, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(spectral, (const char*)c->ch_out.p + ob, ob, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }

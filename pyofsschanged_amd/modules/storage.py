@@ -106,6 +106,7 @@ class Storage:
         field.py:278) and one without any temporal peak a NaN duration."""
         self.z.extend(zs)
         self.As.extend(As)
+        self._ctx = ctx
         if self.domain is None or self.f is None or len(As) == 0:
             return
         ch = ctx.trace_characteristics(np.stack(As), simpson_weights(self.domain.t * 1e-3))
@@ -188,7 +189,24 @@ class Storage:
         return df
 
     def get_plot_data(self, is_temporal=True, reduced_range=None, normalised=False, channel=None):
-        """storage.py:304-350 (without range reduction)."""
+        """storage.py:304-350. With down-sampling (and neither range
+        reduction nor normalisation) the powers are formed and down-sampled
+        on the device, both domains in one launch, and kept for the other
+        domain's call."""
+        ctx = getattr(self, "_ctx", None)
+        if (self.downsampling is not None and reduced_range is None and not normalised and ctx is not None
+                and len(self.As) > 0 and ctx.n == len(self.As[0])):
+            key = (id(self.As), len(self.As), int(self.downsampling))
+            if getattr(self, "_down_key", None) != key:
+                from .._lib import DOWNSAMPLE_PYOFSS
+                self._down = ctx.trace_powers(np.stack(self.As), int(self.downsampling), DOWNSAMPLE_PYOFSS)
+                self._down_key = key
+            if not is_temporal:
+                field.fft_counter += len(self.As)  # spectral_power's field.fft per trace
+            x = self.domain.t if is_temporal else self.domain.nu
+            y = list(self._down[0] if is_temporal else self._down[1])
+            self.plt_data = (x, y, self.z)
+            return self.plt_data
         if is_temporal:
             x = self.domain.t
             power = [temporal_power(A) for A in self.As]

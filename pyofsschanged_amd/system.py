@@ -8,6 +8,7 @@ import pandas as pd
 
 from .domain import Domain
 from .modules.fibre import Fibre
+from .modules.opencl_fibre import OpenclFibre
 from .modules.storage import check_dir
 
 
@@ -106,8 +107,12 @@ class System:
         df = pd.DataFrame()
         z_curr = 0
         for obj in self.modules:
+            df_new = None
             if type(obj) is Fibre:
                 df_new = obj.stepper.storage.get_df(type=df_type, z_curr=z_curr)
+            elif type(obj) is OpenclFibre:
+                df_new = obj.get_df(type=df_type, z_curr=z_curr)
+            if df_new is not None:
                 if not df_new.empty:
                     z_curr = df_new.iloc[-1].name[-1]
                     df = pd.concat([df, df_new])
@@ -125,8 +130,12 @@ class System:
     def update_laser_info(self, obj):
         """system.py:186-199. A fibre that stored no trace (traces=1) adds no
         rows; the reference fails there indexing the empty z axis."""
+        df_new = None
         if type(obj) is Fibre and obj.stepper.storage.z:
             df_new = obj.stepper.storage.get_df_result(self.z_curr)
+        elif type(obj) is OpenclFibre and obj.z_list:
+            df_new = obj.get_df_result(self.z_curr)
+        if df_new is not None:
             self.z_curr = df_new.index.get_level_values("z [mm]").values[-1]
             self.df_results = df_new if self.df_results is None else pd.concat([self.df_results, df_new])
 

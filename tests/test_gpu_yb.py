@@ -169,5 +169,13 @@ def test_published_attractor_run(P):
             cpu = float(gold(col, "cpu")[j][1])
             assert abs(rec[col] - cpu) <= 6e-7 + 5e-9 * abs(cpu) + quant, col
         assert [int(i) for i in rec["Peaks [idx]"]] == json.loads(gold("Peaks [idx]", "cpu")[j][1])
-    # the published run's trace frames: 4000-point down-sampled powers
+    # the published run's trace frames: 4000-point down-sampled powers, formed
+    # on the device, against field.get_downsampled on the host
     assert sys0.df_temp.shape == (500, 4000) and sys0.df_spec.shape == (500, 4000)
+    st = fib.stepper.storage
+    for k in (0, 250, 499):
+        Pt, Ps = P.temporal_power(st.As[k]), P.spectral_power(st.As[k])
+        et, es = P.get_downsampled(Pt, 4000), P.get_downsampled(Ps, 4000)
+        # re^2 + im^2 against abs(A)**2, and the device FFT against numpy's
+        assert np.max(np.abs(sys0.df_temp.values[k] - et)) <= 1e-15 * Pt.max()
+        assert np.max(np.abs(sys0.df_spec.values[k] - es)) <= 4e-15 * Ps.max()
