@@ -115,6 +115,20 @@ int gnlse_set_linear(gnlse_ctx* ctx, const double* factor);
 int gnlse_set_nonlinear(gnlse_ctx* ctx, int kind, double gamma, double f_R, double ss, double rs,
                         const double* omega, const double* nl_mult, const double* h_R);
 
+/* Two-channel WDM fields (pyofss sim_type="wdm"). Linear factors of the
+ * two channels (complex128[n] each, FFT order): Linearity.wdm_linearity,
+ * linearity.py:188-211. Nonlinear coefficients gamma_0, gamma_1 with the
+ * cross-phase coupling N_c = i gamma_c (|A_c|^2 + 2 |A_o|^2) A_c:
+ * Nonlinearity.wdm_nonlinearity / wdm_f / wdm_exp_f, nonlinearity.py:
+ * 194-200, 287-304. After gnlse_set_nonlinear_wdm every field argument of the
+ * operator, step and propagation entry points holds two channels per
+ * instance, [batch][2][n] (traces [batch][max_traces][2][n]); the relative
+ * local error of the adaptive stepper is taken over both channels (the norm
+ * of the 2 x n array, stepper.py:230-245). gnlse_set_nonlinear returns to
+ * single-channel fields. No gain model with WDM. */
+int gnlse_set_linear_wdm(gnlse_ctx* ctx, const double* factor0, const double* factor1);
+int gnlse_set_nonlinear_wdm(gnlse_ctx* ctx, double gamma0, double gamma1);
+
 /* Saturable gain coupled into every linear sub-step. Replaces
  * Amplifier.factor (amplifier.py:149-158) as called from linearity.py:248-272.
  * gain_log = ln(10^(0.1*G_dB)); profile: float64[n] FFT order; simpson_w:

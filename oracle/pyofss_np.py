@@ -6,8 +6,8 @@ Restates, in complex128, the arithmetic of these reference files
   pyofss/field.py:43-172         fft/ifft convention, shifts, powers, energy
   pyofss/domain.py:39-243        grids and unit conversions
   pyofss/modules/sech.py:62-168, gaussian.py:62-175   input pulses
-  pyofss/modules/linearity.py:35-272                  linear operator
-  pyofss/modules/nonlinearity.py:37-270               nonlinear operator
+  pyofss/modules/linearity.py:35-320                  linear operator (incl. WDM)
+  pyofss/modules/nonlinearity.py:37-304               nonlinear operator (incl. WDM)
   pyofss/modules/amplifier.py:42-175                  saturable gain
   pyofss/modules/amplifier.py:178-402                 two-level Yb gain (CPU path)
   pyofss/modules/solver.py:24-354                     integrators
@@ -24,6 +24,9 @@ Differences from the reference, all deliberate:
     AttributeError of stepper.py:365-371 (``storage.dir_spec`` does not exist).
   * Storage keeps (z, A) and the FFT count of update_characts; the other trace
     characteristics are computed only when ``characts=True``.
+  * WDM: Fibre takes no beta_2 from per-channel beta lists (fibre.py:88 fails
+    there) and wdm_linearity subtracts the attenuation per channel (the
+    reference's item assignment into a tuple fails).
 
 Pins (tests/test_oracle_*.py):
   solver + standard stepper  -> bit-exact scalar-ODE values, test_solver.py:83-187
@@ -45,6 +48,9 @@ Pins (tests/test_oracle_*.py):
                                  (first five traces, 1e-11; tests/test_yb_oracle.py),
                                  pump-only inversion closed form
   Hollenbeck-Cantrell Raman   -> the same notebook run (its use_all='hollenbeck')
+  WDM (two channels, XPM)     -> closed-form cross-phase rotation without dispersion,
+                                 closed-form per-channel dispersion without gamma,
+                                 per-channel energy conservation (tests/test_wdm_oracle.py)
   simps_avg                   -> exact on cubics, reduces to scipy.simpson on odd N
   OPPM noise, coherence       -> Parseval, |g12| closed forms (build-only parts)
 """
@@ -589,21 +595,26 @@ def _factorial(n):
 
 
 class Linearity:
-    """linearity.py:106-272 (default sim_type)."""
+    """linearity.py:106-320 (default and wdm sim_type)."""
 
     def __init__(self, alpha=None, beta=None, sim_type=None, use_cache=False, centre_omega=None, phase_lim=False,
                  amplifier=None):
         self.alpha, self.beta, self.centre_omega, self.phase_lim = alpha, beta, centre_omega, phase_lim
         self.amplifier = amplifier
-        self.lin = self.default_f
-        self.exp_lin = self.default_exp_f_cached if use_cache else self.default_exp_f
+        self.wdm = sim_type == "wdm"
+        if self.wdm:  # linearity.py:136-144 getattr dispatch on sim_type
+            self.lin = self.wdm_f
+            self.exp_lin = self.wdm_exp_f_cached if use_cache else self.wdm_exp_f
+        else:
+            self.lin = self.default_f
+            self.exp_lin = self.default_exp_f_cached if use_cache else self.default_exp_f
         self.cached_factor = None
         self.factor = None
         self.Domega = None
 
     def __call__(self, domain):
         self.domain = domain
-        return self.default_linearity(domain)
+        return self.wdm_linearity(domain) if self.wdm else self.default_linearity(domain)
 
     def default_linearity(self, domain):
         """linearity.py:162-186"""
@@ -624,6 +635,54 @@ class Linearity:
             return self.factor
         self.factor -= 0.5 * self.alpha
         return self.factor
+
+    def wdm_linearity(self, domain):
+        """linearity.py:188-211. The attenuation is subtracted per channel
+        (the reference's item assignment into its tuple raises TypeError)."""
+        if self.beta is None:
+            self.factor = (0.0, 0.0)
+        else:
+            if self.centre_omega is None:
+                self.Domega = (domain.omega - domain.centre_omega, domain.omega - domain.centre_omega)
+            else:
+                self.Domega = (domain.omega - self.centre_omega[0], domain.omega - self.centre_omega[1])
+            terms = [0.0, 0.0]
+            for n, beta in enumerate(self.beta[0]):
+                terms[0] += beta * np.power(self.Domega[0], n) / _factorial(n)
+            for n, beta in enumerate(self.beta[1]):
+                terms[1] += beta * np.power(self.Domega[1], n) / _factorial(n)
+            self.factor = (1j * fftshift(terms[0]), 1j * fftshift(terms[1]))
+        if self.alpha is not None:
+            self.factor = (self.factor[0] - 0.5 * self.alpha[0], self.factor[1] - 0.5 * self.alpha[1])
+        return self.factor
+
+    def wdm_cache(self, h):
+        """linearity.py:226-231"""
+        hf0 = self.factor[0] * h
+        hf1 = self.factor[1] * h
+        if self.phase_lim:
+            hf0 = self._limit_imag_part(hf0)
+            hf1 = self._limit_imag_part(hf1)
+        self.cached_factor = [np.exp(hf0), np.exp(hf1)]
+
+    def wdm_f(self, As, z):
+        """linearity.py:303-304"""
+        return np.asarray([ifft(self.factor[0] * fft(As[0])), ifft(self.factor[1] * fft(As[1]))])
+
+    def wdm_exp_f(self, As, h):
+        """linearity.py:306-313"""
+        hf0 = h * self.factor[0]
+        hf1 = h * self.factor[1]
+        if self.phase_lim:
+            hf0 = self._limit_imag_part(hf0)
+            hf1 = self._limit_imag_part(hf1)
+        return np.asarray([ifft(np.exp(hf0) * fft(As[0])), ifft(np.exp(hf1) * fft(As[1]))])
+
+    def wdm_exp_f_cached(self, As, h):
+        """linearity.py:315-320"""
+        if self.cached_factor is None:
+            self.wdm_cache(h)
+        return np.asarray([ifft(self.cached_factor[0] * fft(As[0])), ifft(self.cached_factor[1] * fft(As[1]))])
 
     def _limit_imag_part(self, hf):
         """linearity.py:213-218"""
@@ -707,7 +766,7 @@ def calculate_raman_term_silica(domain):
 
 
 class Nonlinearity:
-    """nonlinearity.py:103-270 (default sim_type)."""
+    """nonlinearity.py:103-304 (default and wdm sim_type)."""
 
     def __init__(self, gamma=None, sim_type=None, self_steepening=False, raman_scattering=False, rs_factor=3e-3,
                  use_all=False, tau_1=12.2e-3, tau_2=32.0e-3, f_R=0.18):
@@ -717,7 +776,10 @@ class Nonlinearity:
         self.rs_factor = rs_factor
         self.use_all = use_all
         self.tau_1, self.tau_2, self.f_R = tau_1, tau_2, f_R
-        if use_all:
+        self.wdm = sim_type == "wdm"
+        if self.wdm and not (use_all or self_steepening or raman_scattering):
+            self.non, self.exp_non = self.wdm_f, self.wdm_exp_f  # nonlinearity.py:148-150 with sim_type wdm
+        elif use_all:
             self.non, self.exp_non = self.default_f_all, self.default_exp_f_all
         elif self_steepening and raman_scattering:
             self.non, self.exp_non = self.default_f_with_ss_and_rs, self.default_exp_f_with_ss_and_rs
@@ -752,7 +814,10 @@ class Nonlinearity:
         else:
             raise UnknownRamanResponseError('Use False, True or a sting from the list ["blowwood", "hollenbeck"]')
         self.h_R *= domain.window_t
-        self.factor = 0.0 if self.gamma is None else 1j * self.gamma
+        if self.wdm:  # nonlinearity.py:194-200
+            self.factor = (0.0, 0.0) if self.gamma is None else (1j * self.gamma[0], 1j * self.gamma[1])
+        else:
+            self.factor = 0.0 if self.gamma is None else 1j * self.gamma
 
     def default_f_all(self, A, z):
         """nonlinearity.py:201-207"""
@@ -807,6 +872,20 @@ class Nonlinearity:
         term_ss = (self.ss_factor / B) * ifft(self.omega * fft(term_spm * A))
         term_rs = self.rs_factor * ifft(1j * self.omega * fft(term_spm))
         return np.exp(h * self.factor * (term_spm + term_ss + term_rs)) * B
+
+    def wdm_f(self, As, z):
+        """nonlinearity.py:287-293"""
+        return np.asarray([
+            self.factor[0] * (np.abs(As[0]) ** 2 + 2.0 * np.abs(As[1]) ** 2) * As[0],
+            self.factor[1] * (np.abs(As[1]) ** 2 + 2.0 * np.abs(As[0]) ** 2) * As[1],
+        ])
+
+    def wdm_exp_f(self, As, h, Bs):
+        """nonlinearity.py:295-304"""
+        return np.asarray([
+            np.exp(h * self.factor[0] * (np.abs(As[0]) ** 2 + 2.0 * np.abs(As[1]) ** 2)) * Bs[0],
+            np.exp(h * self.factor[1] * (np.abs(As[1]) ** 2 + 2.0 * np.abs(As[0]) ** 2)) * Bs[1],
+        ])
 
     def default_f(self, A, z):
         """nonlinearity.py:266-267"""
@@ -1209,7 +1288,8 @@ class Fibre:
         self.name = name
         self.length = length
         self.gamma = gamma
-        self.beta_2 = beta[2] if beta is not None else None
+        # fibre.py:88 fails for WDM's per-channel beta lists; no L_D there
+        self.beta_2 = beta[2] if (beta is not None and sim_type != "wdm") else None
         self.domain = None
         self.amplifier = None
         if use_Yb_model:  # fibre.py:97-98

@@ -88,6 +88,8 @@ SIGNATURES = {
     "gnlse_set_linear": [_P, _P],
     "gnlse_set_nonlinear": [_P, _INT, _D, _D, _D, _D, _P, _P, _P],
     "gnlse_set_gain": [_P, _INT, _D, _D, _D, _P, _P],
+    "gnlse_set_linear_wdm": [_P, _P, _P],
+    "gnlse_set_nonlinear_wdm": [_P, _D, _D],
     "gnlse_set_yb_gain": [_P, _P, _P, _P, _P, _D, _D, _D, _D, _D, _D],
     "gnlse_set_pump": [_P, _I64, _P],
     "gnlse_get_pump": [_P, _I64, _P],
@@ -209,6 +211,7 @@ class Context:
         # Object with a pump power `_Pp` (an Amplifier2LevelModel) whose state
         # every launch with the Yb gain starts from and writes back
         self.pump_source = None
+        self.fields = 1  # fields per instance: 2 after set_nonlinear_wdm
 
     def close(self):
         if getattr(self, "_h", None):
@@ -228,6 +231,8 @@ class Context:
     # -- set-up -------------------------------------------------------------
     def set_grid(self, n):
         self._call("gnlse_set_grid", int(n))
+        if self.n != int(n):
+            self.fields = 1  # a new grid starts with single-channel fields
         self.n = int(n)
 
     def set_linear(self, factor):
@@ -240,12 +245,30 @@ class Context:
         hr = None if h_R is None else _c128(h_R, self.n)
         self._call("gnlse_set_nonlinear", int(kind), float(gamma), float(f_R), float(ss), float(rs), _ptr(om),
                    _ptr(nm), _ptr(hr))
+        self.fields = 1
 
     def set_gain(self, enabled, gain_log=0.0, length=1.0, E_sat=1.0, profile=None, simpson_w=None):
         pr = None if profile is None else _f64(profile, self.n)
         sw = None if simpson_w is None else _f64(simpson_w, self.n)
         self._call("gnlse_set_gain", int(bool(enabled)), float(gain_log), float(length), float(E_sat), _ptr(pr),
                    _ptr(sw))
+
+    def set_linear_wdm(self, factor0, factor1):
+        """Linear factors of the two WDM channels (linearity.py:188-211)."""
+        f0 = _c128(np.broadcast_to(factor0, (self.n,)), self.n)
+        f1 = _c128(np.broadcast_to(factor1, (self.n,)), self.n)
+        self._call("gnlse_set_linear_wdm", _ptr(f0), _ptr(f1))
+
+    def set_nonlinear_wdm(self, gamma0, gamma1):
+        """WDM coupling i gamma_c (|A_c|^2 + 2 |A_o|^2) A_c; from here on every
+        field argument holds two channels per instance."""
+        self._call("gnlse_set_nonlinear_wdm", float(gamma0), float(gamma1))
+        self.fields = 2
+
+    def _instances(self, rows):
+        if rows % self.fields:
+            raise ValueError(f"WDM fields need {self.fields} channels per instance, got {rows} rows")
+        return rows // self.fields
 
     def set_yb_gain(self, ratio_over_psat, inv_psat, alpha_s, eta_s, power_factor, ratio_p, psat_p, n_total,
                     alpha_p, eta_p):
@@ -292,21 +315,22 @@ class Context:
     def apply_linear(self, A, h):
         a, shape = self._batched(A)
         out = np.empty_like(a)
-        self._pump_in(a.shape[0])
-        self._call("gnlse_apply_linear", float(h), a.shape[0], _ptr(a), _ptr(out))
-        self._pump_out(a.shape[0])
+        B = self._instances(a.shape[0])
+        self._pump_in(B)
+        self._call("gnlse_apply_linear", float(h), B, _ptr(a), _ptr(out))
+        self._pump_out(B)
         return out.reshape(shape)
 
     def apply_linear_factor(self, A):
         a, shape = self._batched(A)
         out = np.empty_like(a)
-        self._call("gnlse_apply_linear_factor", a.shape[0], _ptr(a), _ptr(out))
+        self._call("gnlse_apply_linear_factor", self._instances(a.shape[0]), _ptr(a), _ptr(out))
         return out.reshape(shape)
 
     def apply_nonlinear(self, A):
         a, shape = self._batched(A)
         out = np.empty_like(a)
-        self._call("gnlse_apply_nonlinear", a.shape[0], _ptr(a), _ptr(out))
+        self._call("gnlse_apply_nonlinear", self._instances(a.shape[0]), _ptr(a), _ptr(out))
         return out.reshape(shape)
 
     def apply_exp_nonlinear(self, A, h, B):
@@ -315,7 +339,7 @@ class Context:
         if b.shape != a.shape:
             raise ValueError("A and B must have the same shape")
         out = np.empty_like(a)
-        self._call("gnlse_apply_exp_nonlinear", float(h), a.shape[0], _ptr(a), _ptr(b), _ptr(out))
+        self._call("gnlse_apply_exp_nonlinear", float(h), self._instances(a.shape[0]), _ptr(a), _ptr(b), _ptr(out))
         return out.reshape(shape)
 
     def step(self, method, A, h, h_exp=None):
@@ -323,17 +347,21 @@ class Context:
         (default h/2)."""
         a, shape = self._batched(A)
         out = np.empty_like(a)
-        self._pump_in(a.shape[0])
-        self._call("gnlse_step", int(method), float(h), float(h_exp or 0.0), a.shape[0], _ptr(a), _ptr(out))
-        self._pump_out(a.shape[0])
+        B = self._instances(a.shape[0])
+        self._pump_in(B)
+        self._call("gnlse_step", int(method), float(h), float(h_exp or 0.0), B, _ptr(a), _ptr(out))
+        self._pump_out(B)
         return out.reshape(shape)
 
     # -- propagation ------------------------------------------------------------
     def propagate_fixed(self, method, A, h, nsteps, trace_every=0, max_traces=0, h_exp=None):
+        """Traces come back as [B, max_traces, N] ([B, max_traces, 2, N] for
+        WDM fields)."""
         a, shape = self._batched(A)
-        B = a.shape[0]
+        B = self._instances(a.shape[0])
         out = np.empty_like(a)
-        traces = np.empty((B, max(max_traces, 0), self.n), np.complex128)
+        traces = np.empty((B, max(max_traces, 0)) + ((self.fields,) if self.fields > 1 else ()) + (self.n,),
+                          np.complex128)
         ntr = np.zeros(B, np.int64)
         self._pump_in(B)
         self._call("gnlse_propagate_fixed", int(method), B, float(h), float(h_exp or 0.0), int(nsteps), int(trace_every),
@@ -344,9 +372,9 @@ class Context:
 
     def propagate_adaptive(self, method, A, params, max_traces=0, log_cap=0, replay=None):
         a, shape = self._batched(A)
-        B = a.shape[0]
+        B = self._instances(a.shape[0])
         out = np.empty_like(a)
-        tA = np.empty((B, max_traces, self.n), np.complex128)
+        tA = np.empty((B, max_traces) + ((self.fields,) if self.fields > 1 else ()) + (self.n,), np.complex128)
         tz = np.zeros((B, max_traces))
         sz = np.zeros((B, max_traces))
         sh = np.zeros((B, max_traces))

@@ -157,7 +157,7 @@ int gnlse_ctx_destroy(gnlse_ctx* c) {
                     &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
                     &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr,
                     &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk, &c->yb_rs, &c->yb_ip, &c->yb_as, &c->yb_es,
-                    &c->yb_pp};
+                    &c->yb_pp, &c->lin2, &c->efix2};
   for (DevBuf* b : bufs) release(*b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -174,7 +174,7 @@ int gnlse_set_grid(gnlse_ctx* c, int64_t n) {
   if (c->n != n) {
     c->n = n;
     c->tab = t;
-    c->have_lin = c->have_nl = false;
+    c->have_lin = c->have_nl = c->have_lin2 = false;
     c->efix_h = NAN;
     RET_IF(build_twiddles(c));
     std::vector<double> zeros(2 * n, 0.0);  // every operator pointer stays valid
@@ -185,6 +185,7 @@ int gnlse_set_grid(gnlse_ctx* c, int64_t n) {
     RET_IF(upload(c, c->simpw, zeros.data(), n * sizeof(double)));
     RET_IF(ensure(c->efix, 2 * n * sizeof(double)));
     c->ops.gain = GAIN_NONE;
+    c->ops.wdm = 0;
     refresh_ops(c);
   }
   return GNLSE_OK;
@@ -196,6 +197,7 @@ int gnlse_set_linear(gnlse_ctx* c, const double* factor) {
   HIPCHK(hipSetDevice(c->device));
   RET_IF(upload(c, c->lin, factor, 2 * c->n * sizeof(double)));
   c->have_lin = true;
+  c->have_lin2 = false;
   c->efix_h = NAN;
   refresh_ops(c);
   return GNLSE_OK;
@@ -214,6 +216,7 @@ int gnlse_set_nonlinear(gnlse_ctx* c, int kind, double gamma, double f_R, double
   if (nl_mult) RET_IF(upload(c, c->nlmul, nl_mult, 2 * n * sizeof(double)));
   if (h_R) RET_IF(upload(c, c->hr, h_R, 2 * n * sizeof(double)));
   c->ops.nl_kind = kind;
+  c->ops.wdm = 0;  // single-channel fields again
   c->ops.fac = make_double2(0.0, gamma);
   c->ops.f_R = f_R;
   c->ops.ss = ss;
@@ -232,6 +235,7 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
     return GNLSE_OK;
   }
   if (!profile || !simpson_w) return fail(GNLSE_ERR_ARG, "profile and simpson_w are required");
+  if (c->ops.wdm) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields carry no gain model");
   if (!(E_sat > 0.0) || !(length > 0.0)) return fail(GNLSE_ERR_ARG, "E_sat and length must be positive");
   const size_t n = (size_t)c->n;
   RET_IF(upload(c, c->gprof, profile, n * sizeof(double)));
@@ -244,11 +248,38 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
   return GNLSE_OK;
 }
 
+int gnlse_set_linear_wdm(gnlse_ctx* c, const double* factor0, const double* factor1) {
+  RET_IF(check_ready(c, false, false));
+  if (!factor0 || !factor1) return fail(GNLSE_ERR_ARG, "null factor");
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(upload(c, c->lin, factor0, 2 * (size_t)c->n * sizeof(double)));
+  RET_IF(upload(c, c->lin2, factor1, 2 * (size_t)c->n * sizeof(double)));
+  RET_IF(ensure(c->efix2, 2 * (size_t)c->n * sizeof(double)));
+  c->have_lin = true;
+  c->have_lin2 = true;
+  c->efix_h = NAN;
+  refresh_ops(c);
+  return GNLSE_OK;
+}
+
+int gnlse_set_nonlinear_wdm(gnlse_ctx* c, double gamma0, double gamma1) {
+  RET_IF(check_ready(c, false, false));
+  if (c->ops.gain != GAIN_NONE) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields carry no gain model");
+  c->ops.wdm = 1;
+  c->ops.nl_kind = NL_KERR;
+  c->ops.fac = make_double2(0.0, gamma0);
+  c->ops.fac2 = make_double2(0.0, gamma1);
+  c->have_nl = true;
+  c->efix_h = NAN;
+  return GNLSE_OK;
+}
+
 int gnlse_set_yb_gain(gnlse_ctx* c, const double* ratio_over_psat, const double* inv_psat, const double* alpha_s,
                       const double* eta_s, double power_factor, double ratio_p, double psat_p, double n_total,
                       double alpha_p, double eta_p) {
   RET_IF(check_ready(c, false, false));
   if (!ratio_over_psat || !inv_psat || !alpha_s || !eta_s) return fail(GNLSE_ERR_ARG, "null Yb array");
+  if (c->ops.wdm) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields carry no gain model");
   if (!(psat_p > 0.0) || !std::isfinite(power_factor) || !std::isfinite(n_total))
     return fail(GNLSE_ERR_ARG, "invalid Yb constants");
   const size_t n = (size_t)c->n;
@@ -306,48 +337,52 @@ int gnlse_apply_linear(gnlse_ctx* c, double h, int64_t batch, const double* in, 
   RET_IF(check_ready(c, true, false));
   RET_IF(check_pump(c, batch));
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(io_prepare(c, batch, in, c->io_in));
-  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  const int64_t rows = batch * fields_per_instance(c);
+  RET_IF(io_prepare(c, rows, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)rows * c->n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   RET_IF(timed_end(c, c->tab->linear(c->stream, c->ops, h, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p,
                                      (double2*)c->ws.p)));
-  return io_finish(c, batch, out);
+  return io_finish(c, rows, out);
 }
 
 int gnlse_apply_linear_factor(gnlse_ctx* c, int64_t batch, const double* in, double* out) {
   RET_IF(check_ready(c, true, false));
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(io_prepare(c, batch, in, c->io_in));
-  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  const int64_t rows = batch * fields_per_instance(c);
+  RET_IF(io_prepare(c, rows, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)rows * c->n * sizeof(double2)));
   RET_IF(timed_begin(c));
   RET_IF(timed_end(c, c->tab->linfac(c->stream, c->ops, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p)));
-  return io_finish(c, batch, out);
+  return io_finish(c, rows, out);
 }
 
 int gnlse_apply_nonlinear(gnlse_ctx* c, int64_t batch, const double* in, double* out) {
   RET_IF(check_ready(c, false, true));
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(io_prepare(c, batch, in, c->io_in));
-  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  const int64_t rows = batch * fields_per_instance(c);
+  RET_IF(io_prepare(c, rows, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)rows * c->n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   RET_IF(timed_end(c, c->tab->nonlin(c->stream, c->ops, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p,
                                      (double2*)c->ws.p)));
-  return io_finish(c, batch, out);
+  return io_finish(c, rows, out);
 }
 
 int gnlse_apply_exp_nonlinear(gnlse_ctx* c, double h, int64_t batch, const double* A, const double* B, double* out) {
   RET_IF(check_ready(c, false, true));
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(io_prepare(c, batch, A, c->io_in));
-  RET_IF(io_prepare(c, batch, B, c->io_b));
-  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  const int64_t rows = batch * fields_per_instance(c);
+  RET_IF(io_prepare(c, rows, A, c->io_in));
+  RET_IF(io_prepare(c, rows, B, c->io_b));
+  RET_IF(ensure(c->io_out, (size_t)rows * c->n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   RET_IF(timed_end(c, c->tab->exp_nonlin(c->stream, c->ops, h, batch, (const double2*)c->io_in.p,
                                          (const double2*)c->io_b.p, (double2*)c->io_out.p, (double2*)c->ws.p)));
-  return io_finish(c, batch, out);
+  return io_finish(c, rows, out);
 }
 
 int gnlse_step(gnlse_ctx* c, int method, double h, double h_exp, int64_t batch, const double* in, double* out) {
@@ -355,13 +390,14 @@ int gnlse_step(gnlse_ctx* c, int method, double h, double h_exp, int64_t batch, 
   RET_IF(check_pump(c, batch));
   if (!method_ok(method)) return fail(GNLSE_ERR_ARG, "unknown method");
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(io_prepare(c, batch, in, c->io_in));
-  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  const int64_t rows = batch * fields_per_instance(c);
+  RET_IF(io_prepare(c, rows, in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)rows * c->n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   RET_IF(timed_end(c, c->tab->step(c->stream, c->ops, method, h, h_exp > 0.0 ? h_exp : 0.5 * h, batch, (const double2*)c->io_in.p,
                                    (double2*)c->io_out.p, (double2*)c->ws.p)));
-  return io_finish(c, batch, out);
+  return io_finish(c, rows, out);
 }
 
 int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, double h_exp, int64_t nsteps,
@@ -373,16 +409,21 @@ int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, dou
   if (max_traces > 0 && (!traces || !n_traces)) return fail(GNLSE_ERR_ARG, "null trace outputs");
   RET_IF(check_pump(c, batch));
   HIPCHK(hipSetDevice(c->device));
-  const int64_t n = c->n;
+  const int64_t n = c->n * fields_per_instance(c);  // field length of an instance
   // linearity.py:220-224, 258-262: the cached factor exp(factor*h_exp), where
   // h_exp is the step of the first linear call (h/2 for both integrators)
   const double hh = h_exp > 0.0 ? h_exp : 0.5 * h;
   if (!(c->efix_h == hh)) {
     hipError_t e = c->tab->make_exp(c->stream, c->ops, hh, (double2*)c->efix.p);
+    if (e == hipSuccess && c->ops.wdm) {  // channel 1's factor (linearity.py:220-231 wdm_cache)
+      Ops o1 = c->ops;
+      o1.lin = c->ops.lin2;
+      e = c->tab->make_exp(c->stream, o1, hh, (double2*)c->efix2.p);
+    }
     if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("make_exp: ") + hipGetErrorString(e));
     c->efix_h = hh;
   }
-  RET_IF(io_prepare(c, batch, A_in, c->io_in));
+  RET_IF(io_prepare(c, batch * fields_per_instance(c), A_in, c->io_in));
   RET_IF(ensure(c->io_out, (size_t)batch * n * sizeof(double2)));
   RET_IF(ensure_ws(c, batch));
   RET_IF(ensure(c->tr_A, (size_t)batch * max_traces * n * sizeof(double2)));
@@ -402,7 +443,7 @@ int gnlse_propagate_fixed(gnlse_ctx* c, int method, int64_t batch, double h, dou
                           c->stream));
     HIPCHK(hipMemcpyAsync(n_traces, c->ntr.p, (size_t)batch * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   }
-  return io_finish(c, batch, A_out);
+  return io_finish(c, batch * fields_per_instance(c), A_out);
 }
 
 static int adaptive_params(int method, const gnlse_adaptive_params* p, AdaptiveParams& prm) {
@@ -451,9 +492,10 @@ int gnlse_propagate_adaptive(gnlse_ctx* c, int method, int64_t batch, const gnls
     return fail(GNLSE_ERR_ARG, "null trace outputs");
   if (log_cap > 0 && (!log_h || !log_delta || !log_accept)) return fail(GNLSE_ERR_ARG, "null log outputs");
   if (replay_h && (!replay_accept || replay_len <= 0)) return fail(GNLSE_ERR_ARG, "invalid replay");
+  if (replay_h && c->ops.wdm) return fail(GNLSE_ERR_UNSUPPORTED, "step replay is not available for WDM fields");
   HIPCHK(hipSetDevice(c->device));
-  const int64_t n = c->n;
-  RET_IF(io_prepare(c, batch, A_in, c->io_in));
+  const int64_t n = c->n * fields_per_instance(c);  // field length of an instance
+  RET_IF(io_prepare(c, batch * fields_per_instance(c), A_in, c->io_in));
   RET_IF(ensure(c->io_out, (size_t)batch * n * sizeof(double2)));
   RET_IF(ensure(c->tr_A, (size_t)batch * max_traces * n * sizeof(double2)));
   RET_IF(ensure(c->tr_z, (size_t)batch * max_traces * sizeof(double)));
@@ -498,7 +540,7 @@ int gnlse_propagate_adaptive(gnlse_ctx* c, int method, int64_t batch, const gnls
     RET_IF(d2h(log_accept, c->lg_a.p, (size_t)batch * log_cap));
   }
   RET_IF(d2h(results, c->res.p, (size_t)batch * sizeof(gnlse_instance_result)));
-  return io_finish(c, batch, A_out);
+  return io_finish(c, batch * fields_per_instance(c), A_out);
 }
 
 int gnlse_malloc(gnlse_ctx* c, int64_t bytes, void** dptr) {
@@ -717,6 +759,7 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
       return fail(GNLSE_ERR_UNSUPPORTED, "ensemble grids are 4096, 8192 or 16384 samples, got " + std::to_string(n));
     if (ctxs[i]->device != ctxs[0]->device) return fail(GNLSE_ERR_ARG, "ensemble groups must share one device");
     if (batch[i] < 1 || batch[i] > (1 << 24)) return fail(GNLSE_ERR_ARG, "group batch out of range");
+    if (ctxs[i]->ops.wdm) return fail(GNLSE_ERR_UNSUPPORTED, "ensembles hold single-channel fields");
     if (ctxs[i]->ops.gain == GAIN_YB)
       return fail(GNLSE_ERR_UNSUPPORTED, "ensembles do not carry the Yb pump state; propagate such groups directly");
   }

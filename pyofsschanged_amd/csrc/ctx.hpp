@@ -50,14 +50,18 @@ struct gnlse_ctx {
   const LaunchTable* tab = nullptr;
   DevBuf lin, efix, nlmul, hr, omega, gprof, simpw, tw;
   double efix_h = NAN;
-  bool have_lin = false, have_nl = false;
+  bool have_lin = false, have_nl = false, have_lin2 = false;
   Ops ops{};
   DevBuf ws, io_in, io_out, io_b, tr_A, tr_z, st_z, st_h, lg_h, lg_d, lg_a, rp_h, rp_a, res, ntr;
   DevBuf ch_w, ch_out, ch_npk, ch_pk;  // trace characteristics
   DevBuf yb_rs, yb_ip, yb_as, yb_es;   // two-level Yb gain arrays
   DevBuf yb_pp;                        // pump power per instance
   int64_t yb_count = 0;                // instances with a pump power set
+  DevBuf lin2, efix2;                  // WDM channel 1 linear factor and its fixed-step exponential
 };
+
+// Fields per instance: 2 with the WDM channels, else 1.
+inline int64_t fields_per_instance(const gnlse_ctx* c) { return c->ops.wdm ? 2 : 1; }
 
 inline int ensure(DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
@@ -117,6 +121,8 @@ inline void refresh_ops(gnlse_ctx* c) {
   c->ops.yb_as = (const double*)c->yb_as.p;
   c->ops.yb_es = (const double*)c->yb_es.p;
   c->ops.yb_pp = (double*)c->yb_pp.p;
+  c->ops.lin2 = (const double2*)c->lin2.p;
+  c->ops.efix2 = (const double2*)c->efix2.p;
 }
 
 // With the Yb gain every instance of a launch carries a pump power.
@@ -131,10 +137,12 @@ inline int check_ready(gnlse_ctx* c, bool need_lin, bool need_nl) {
   if (c->n == 0 || !c->tab) return fail(GNLSE_ERR_STATE, "gnlse_set_grid has not been called");
   if (need_lin && !c->have_lin) return fail(GNLSE_ERR_STATE, "gnlse_set_linear has not been called");
   if (need_nl && !c->have_nl) return fail(GNLSE_ERR_STATE, "gnlse_set_nonlinear has not been called");
+  if (need_lin && c->ops.wdm && !c->have_lin2)
+    return fail(GNLSE_ERR_STATE, "WDM fields need both channels' linear factors (gnlse_set_linear_wdm)");
   return GNLSE_OK;
 }
 
 inline int ensure_ws(gnlse_ctx* c, int64_t batch) {
-  return ensure(c->ws, (size_t)batch * NSLOT * c->n * sizeof(double2));
+  return ensure(c->ws, (size_t)batch * fields_per_instance(c) * NSLOT * c->n * sizeof(double2));
 }
 

@@ -373,6 +373,12 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
     else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false>), __VA_ARGS__);                                 \
   } while (0)
 
+}  // namespace gn
+
+#include "wdm.hpp"
+
+namespace gn {
+
 template <int N, int T>
 struct Launcher {
   static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out) {
@@ -385,39 +391,55 @@ struct Launcher {
   }
   static hipError_t linear(hipStream_t s, const Ops& o, double h, long long batch, const double2* in, double2* out,
                            double2* ws) {
-    if (o.gain == GAIN_YB) hipLaunchKernelGGL((k_linear<N, T, true>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
+    if (o.wdm) hipLaunchKernelGGL((k_linear_wdm<N, T>), dim3(2 * batch), dim3(T), 0, s, o, h, in, out, ws);
+    else if (o.gain == GAIN_YB) hipLaunchKernelGGL((k_linear<N, T, true>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
     else hipLaunchKernelGGL((k_linear<N, T, false>), dim3(batch), dim3(T), 0, s, o, h, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t linfac(hipStream_t s, const Ops& o, long long batch, const double2* in, double2* out) {
-    hipLaunchKernelGGL((k_linfac<N, T>), dim3(batch), dim3(T), 0, s, o, in, out);
+    if (o.wdm) hipLaunchKernelGGL((k_linfac_wdm<N, T>), dim3(2 * batch), dim3(T), 0, s, o, in, out);
+    else hipLaunchKernelGGL((k_linfac<N, T>), dim3(batch), dim3(T), 0, s, o, in, out);
     return hipGetLastError();
   }
   static hipError_t nonlin(hipStream_t s, const Ops& o, long long batch, const double2* in, double2* out,
                            double2* ws) {
-    GN_BY_KIND(o, k_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, in, out, ws);
+    if (o.wdm) hipLaunchKernelGGL((k_nonlin_wdm<N, T>), dim3(batch), dim3(T), 0, s, o, 0.0, in, nullptr, out);
+    else GN_BY_KIND(o, k_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t exp_nonlin(hipStream_t s, const Ops& o, double h, long long batch, const double2* A,
                                const double2* B, double2* out, double2* ws) {
-    GN_BY_KIND(o, k_exp_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, h, A, B, out, ws);
+    if (o.wdm) hipLaunchKernelGGL((k_nonlin_wdm<N, T>), dim3(batch), dim3(T), 0, s, o, h, A, B, out);
+    else GN_BY_KIND(o, k_exp_nonlin<N COMMA T, dim3(batch), dim3(T), 0, s, o, h, A, B, out, ws);
     return hipGetLastError();
   }
   static hipError_t step(hipStream_t s, const Ops& o, int method, double h, double h_exp, long long batch, const double2* in,
                          double2* out, double2* ws) {
-    if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    if (o.wdm && method == M_RK4IP)
+      hipLaunchKernelGGL((k_step_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    else if (o.wdm)
+      hipLaunchKernelGGL((k_step_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
+    else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_step<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_step<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t fixed(hipStream_t s, const Ops& o, int method, const FixedParams& p, long long batch,
                           const double2* in, double2* out, double2* ws) {
-    if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    if (o.wdm && method == M_RK4IP)
+      hipLaunchKernelGGL((k_fixed_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (o.wdm)
+      hipLaunchKernelGGL((k_fixed_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t adaptive(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
                              const double2* in, double2* out, double2* ws) {
-    if (method == M_RK4IP)
+    if (o.wdm && method == M_RK4IP)
+      hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (o.wdm)
+      hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP)
       GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();

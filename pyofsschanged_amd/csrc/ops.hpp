@@ -54,6 +54,12 @@ struct Ops {
   double yb_n;              // total ion number N
   double yb_ap, yb_ep;      // alpha_p, eta_p
   double* yb_pp;            // pump power per instance (read at start, written at end)
+  // two-channel WDM (linearity.py:188-211, 306-320; nonlinearity.py:197-200,
+  // 287-304): channel 0 uses lin/efix/fac, channel 1 these
+  int wdm;
+  const double2* lin2;      // channel 1 linear factor, FFT order
+  const double2* efix2;     // exp(lin2 * h/2) of the fixed-step path
+  double2 fac2;             // i*gamma_1
 };
 
 __device__ __forceinline__ double2 cexp(double2 z) {
@@ -384,10 +390,12 @@ struct Dev {
   }
 
   // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
-  __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const {
+  __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const { make_exp_of(E, op.lin, h); }
+  // exp(factor * h) of a given linear factor (the WDM channels have two)
+  __device__ __forceinline__ void make_exp_of(double2* __restrict__ E, const double2* __restrict__ fac, double h) const {
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      const double2 f = op.lin[idx(k)];
+      const double2 f = fac[idx(k)];
       E[idx(k)] = cexp(make_double2(f.x * h, f.y * h));
     }
   }

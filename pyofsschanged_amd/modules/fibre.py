@@ -33,7 +33,10 @@ class Fibre:
         self.name = name
         self.length = length
         self.small_signal_gain = small_signal_gain
-        self.beta_2 = beta[2] if beta is not None else None
+        self.wdm = sim_type == "wdm"
+        # fibre.py:88; with WDM's per-channel beta lists the reference fails
+        # there (beta[2] of a two-element list): no dispersion length then
+        self.beta_2 = beta[2] if (beta is not None and not self.wdm) else None
         self.gamma = gamma
         self.L_D = None
         self.L_NL = None
@@ -102,8 +105,8 @@ class Fibre:
     def _count_ffts(self, method, calls):
         """Advance field.fft_counter as pyofss counts FFTs (field.py:118-134)."""
         nl = _NL_FFTS[self.nonlinearity.kind]
-        g = self.linearity.gain_ffts()  # per linear call: 4 in rk4ip, 2 in ss_symmetric
-        per_call = 8 + 4 * nl + 4 * g if method == _lib.METHOD_RK4IP else 4 + nl + 2 * g
+        lin = 2 * self.linearity.channels() + self.linearity.gain_ffts()  # per linear call
+        per_call = 4 * lin + 4 * nl if method == _lib.METHOD_RK4IP else 2 * lin + nl
         field.fft_counter += per_call * calls
 
     def prepare(self, domain):
@@ -139,7 +142,11 @@ class Fibre:
         raise NotImplementedError("step_amplifier is not supported by the HIP engine")
 
     def calculate_reference_length(self, field_in):
-        """fibre.py:169-177"""
+        """fibre.py:169-177 (no reference length for WDM fields, whose
+        per-channel gamma and beta the reference cannot evaluate here)."""
+        if self.wdm:
+            self.L_D = self.L_NL = self.reference_length = None
+            return
         self.L_D = self.get_dispersion_length(field_in)
         self.L_NL = self.get_nonlinear_length(field_in)
         if self.L_NL and self.L_D is not None:

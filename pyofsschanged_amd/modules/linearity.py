@@ -47,8 +47,11 @@ class Linearity:
 
     def __init__(self, alpha=None, beta=None, sim_type=None, use_cache=False, centre_omega=None, phase_lim=False,
                  amplifier=None, use_Er_noise=False):
-        if sim_type not in (None, "default"):
+        if sim_type not in (None, "default", "wdm"):
             raise NotImplementedError(f"sim_type={sim_type!r} is not supported by the HIP engine")
+        self.wdm = sim_type == "wdm"
+        if self.wdm and amplifier is not None:
+            raise NotImplementedError("WDM fields carry no gain model (linearity.py:306-320 ignores the amplifier)")
         if use_Er_noise:
             raise NotImplementedError("Er ASE noise is not supported by the HIP engine")
         self.alpha = alpha
@@ -58,7 +61,7 @@ class Linearity:
         self.amplifier = amplifier
         self.use_Er_noise = use_Er_noise
         self.use_cache = use_cache
-        self.generate_linearity = self.default_linearity
+        self.generate_linearity = self.wdm_linearity if self.wdm else self.default_linearity
         self.generate_cache = self.default_cache
         self.lin = self.default_f
         self.exp_lin = self.default_exp_f_cached if use_cache else self.default_exp_f
@@ -93,6 +96,28 @@ class Linearity:
         self._upload()
         return self.factor
 
+    def wdm_linearity(self, domain):
+        """linearity.py:188-211; the attenuation is subtracted from each
+        channel's factor (the reference's in-place update of its tuple
+        raises TypeError)."""
+        if self.beta is None:
+            self.factor = (0.0, 0.0)
+        else:
+            if self.centre_omega is None:
+                self.Domega = (domain.omega - domain.centre_omega, domain.omega - domain.centre_omega)
+            else:
+                self.Domega = (domain.omega - self.centre_omega[0], domain.omega - self.centre_omega[1])
+            terms = [0.0, 0.0]
+            for c in range(2):
+                for n, beta in enumerate(self.beta[c]):
+                    terms[c] += beta * np.power(self.Domega[c], n) / float(factorial(n))
+            self.factor = (1j * fftshift(terms[0]), 1j * fftshift(terms[1]))
+        if self.alpha is not None:
+            self.factor = (self.factor[0] - 0.5 * self.alpha[0], self.factor[1] - 0.5 * self.alpha[1])
+        self.cached_factor = None
+        self._upload()
+        return self.factor
+
     # -- device binding -----------------------------------------------------------
     def bind(self, ctx):
         """Use engine context ``ctx`` (shared with a Fibre's Nonlinearity)."""
@@ -109,7 +134,12 @@ class Linearity:
 
     def _upload(self):
         ctx = self._context()
-        ctx.set_linear(np.broadcast_to(np.asarray(self.factor, dtype=np.complex128), (self.domain.total_samples,)))
+        n = self.domain.total_samples
+        if self.wdm:
+            ctx.set_linear_wdm(*[np.broadcast_to(np.asarray(f, dtype=np.complex128), (n,)) for f in self.factor])
+            ctx.pump_source = None
+            return
+        ctx.set_linear(np.broadcast_to(np.asarray(self.factor, dtype=np.complex128), (n,)))
         ctx.pump_source = None
         if self._yb():
             ctx.set_yb_gain(*self.amplifier.device_arrays())
@@ -122,6 +152,9 @@ class Linearity:
     def _yb(self):
         from .amplifier import Amplifier2LevelModel
         return isinstance(self.amplifier, Amplifier2LevelModel)
+
+    def channels(self):
+        return 2 if self.wdm else 1
 
     def gain_ffts(self):
         """FFTs pyofss counts inside the amplifier per exp_lin call: the Yb
@@ -136,18 +169,18 @@ class Linearity:
         self.generate_cache(h)
 
     def default_f(self, A, z):
-        """linearity.py:238-239: ifft(factor * fft(A)) on the GPU."""
-        field.fft_counter += 2
+        """linearity.py:238-239 (wdm: :303-304): ifft(factor * fft(A)) on the GPU."""
+        field.fft_counter += 2 * self.channels()
         return self._context().apply_linear_factor(A)
 
     def default_exp_f(self, A, h):
         """linearity.py:241-256: ifft(exp(factor*h) [* exp(gain)] * fft(A)) on the GPU."""
-        field.fft_counter += 2 + self.gain_ffts()
+        field.fft_counter += 2 * self.channels() + self.gain_ffts()
         return self._context().apply_linear(A, h)
 
     def default_exp_f_cached(self, A, h):
         """linearity.py:258-272: as default_exp_f with the step of the first call."""
         if self.cached_factor is None:
             self.cache(h)
-        field.fft_counter += 2 + self.gain_ffts()
+        field.fft_counter += 2 * self.channels() + self.gain_ffts()
         return self._context().apply_linear(A, self.cached_factor)

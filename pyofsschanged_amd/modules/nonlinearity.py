@@ -59,12 +59,17 @@ _NL_FFTS = {_lib.NL_KERR: 0, _lib.NL_ALL: 4, _lib.NL_SS: 2, _lib.NL_RS: 2, _lib.
 
 
 class Nonlinearity:
-    """nonlinearity.py:103-270 (default sim_type)."""
+    """nonlinearity.py:103-304 (default and wdm sim_type)."""
 
     def __init__(self, gamma=None, sim_type=None, self_steepening=False, raman_scattering=False, rs_factor=3e-3,
                  use_all=False, tau_1=12.2e-3, tau_2=32.0e-3, f_R=0.18):
-        if sim_type not in (None, "default"):
+        if sim_type not in (None, "default", "wdm"):
             raise NotImplementedError(f"sim_type={sim_type!r} is not supported by the HIP engine")
+        self.wdm = sim_type == "wdm"
+        if self.wdm and (use_all or self_steepening or raman_scattering):
+            # nonlinearity.py:272-285: the reference's WDM variants with these
+            # flags return None (use_all falls back to the single-field operator)
+            raise NotImplementedError("WDM fields support the Kerr and cross-phase terms only")
         self.gamma = gamma
         self.self_steepening = self_steepening
         self.raman_scattering = raman_scattering
@@ -73,7 +78,7 @@ class Nonlinearity:
         self.tau_1 = tau_1
         self.tau_2 = tau_2
         self.f_R = f_R
-        self.generate_nonlinearity = self.default_nonlinearity
+        self.generate_nonlinearity = self.wdm_nonlinearity if self.wdm else self.default_nonlinearity
         # nonlinearity.py:133-152 flag dispatch
         if use_all:
             print("Using general expression for nonlinearity")
@@ -126,6 +131,10 @@ class Nonlinearity:
         """nonlinearity.py:187-192"""
         self.factor = 0.0 if self.gamma is None else 1j * self.gamma
 
+    def wdm_nonlinearity(self):
+        """nonlinearity.py:194-200"""
+        self.factor = (0.0, 0.0) if self.gamma is None else (1j * self.gamma[0], 1j * self.gamma[1])
+
     # -- device binding -----------------------------------------------------------
     def bind(self, ctx):
         self._ctx = ctx
@@ -140,6 +149,10 @@ class Nonlinearity:
 
     def _upload(self):
         ctx = self._context()
+        if self.wdm:  # i gamma_c (|A_c|^2 + 2|A_o|^2) A_c, nonlinearity.py:287-304
+            g = (0.0, 0.0) if self.gamma is None else (float(self.gamma[0]), float(self.gamma[1]))
+            ctx.set_nonlinear_wdm(g[0], g[1])
+            return
         gamma = 0.0 if self.gamma is None else float(self.gamma)
         nl_mult = h_R = None
         if self.kind == _lib.NL_ALL:
