@@ -193,10 +193,14 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // instance b of a batch whose inputs, outputs and workspaces start at in,
 // out and ws_all. With spec != nullptr the pyofss spectrum fft(A_out) is
 // also written to spec + b*N (input of the ensemble coherence).
-template <int N, int T, int METHOD, int K, bool YB = false>
+// SP selects rk4ip_sp (RK4IP, full nonlinear operator, no gain): the
+// accepted field is then also kept as its spectrum in S_T and the half
+// step's spectrum, left in S_HS, starts the fine step.
+template <int N, int T, int METHOD, int K, bool YB = false, bool SP = false>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
                                               double2* ws_all, double2* __restrict__ spec) {
+  static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && !YB), "spectral-domain stepping: rk4ip, NL_ALL, no gain");
   Dev<N, T, K, YB> d(lds, red, ops);
   constexpr int P = N / T;
   double2* ws = ws_all + b * NSLOT * N;
@@ -204,6 +208,8 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   double2* sH = ws + S_H * N;
   double2* sEQ = ws + S_EQ * N;
   double2* sEH = ws + S_EH * N;
+  double2* sAs = ws + S_T * N;   // SP: fft(A)
+  double2* sHs = ws + S_HS * N;  // SP: fft of the half step's result
   double2 x[P];
   d.load(x, in + b * N);
   d.store(sA, x);
@@ -231,6 +237,10 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   };
 
   if (prm.traces != 1 && prm.max_traces > 0) store_trace(0.0, 0.0);  // stepper.py:284-285
+  if constexpr (SP) {
+    d.fft(x);
+    d.store(sAs, x);
+  }
 
   for (long long s = 1; s < prm.steps_max; ++s) {
     if ((z + h) > prm.length) h = prm.length - z;  // stepper.py:291-292
@@ -256,7 +266,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         // N(A) is shared by the half and the coarse step, and by every
         // attempt from the same accepted field
         const bool from_a = (c != 1);
-        integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
+        if constexpr (SP)
+          rk4ip_sp(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
+                   from_a && nA_valid, from_a && !nA_valid);
+        else
+          integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
         if (from_a) nA_valid = true;
         if (c < 2) d.store(sH, x);
       }
@@ -306,6 +320,10 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
           if (prm.max_traces > 0 && !store_trace(z, h_temp)) status = GNLSE_STATUS_TRACE_OVERFLOW;
           z_ignore += prm.step_storage;
         }
+        if constexpr (SP) {
+          d.fft(x);
+          d.store(sAs, x);
+        }
         accepted = true;
         break;
       } else if (h < prm.hmin) {  // stepper.py:349-351
@@ -344,12 +362,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
 }
 
-template <int N, int T, int METHOD, int K, bool YB>
+template <int N, int T, int METHOD, int K, bool YB, bool SP = false>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
   __shared__ double lds[FFT<N, T>::LDS_DOUBLES];
   __shared__ double red[T / 64 + 2];
-  adaptive_body<N, T, METHOD, K, YB>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
+  adaptive_body<N, T, METHOD, K, YB, SP>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
@@ -439,6 +457,8 @@ struct Launcher {
       hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.wdm)
       hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.gain == GAIN_NONE && o.nl_kind == NL_ALL)
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, true>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP)
       GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);

@@ -90,6 +90,93 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
   return make_double2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
+// Pointwise passes that load and store different arrays. Each pointer is
+// offset to the calling thread's first owned point and the points follow at
+// strides of T. The compiler keeps loads below earlier stores unless it can
+// prove they differ, which it does not here, so a pass written point by
+// point waits for one memory round trip per point; these passes load a
+// block of C points of every input first, then compute and store it.
+template <int P, int T>
+__device__ __forceinline__ void pw_exp(double2* E, const double2* fac, double h) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double2 f = fac[T * k];
+    E[T * k] = cexp(make_double2(f.x * h, f.y * h));
+  }
+}
+// rk4ip (solver.py:353): acc = A_I + (k0 + 2 (k1 + k2))/6 into K0, and
+// x <- A_I + k2 (x holds k2 on entry).
+template <int P, int T>
+__device__ __forceinline__ void pw_rk4_acc(double2 (&x)[P], const double2* AI, double2* K0, const double2* K1) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double2 ai = AI[T * k];
+    const double2 s = cadd(K0[T * k], cscale(cadd(K1[T * k], x[k]), 2.0));
+    K0[T * k] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));
+    x[k] = cadd(ai, x[k]);
+  }
+}
+// Entry of rk4ip_sp: A_I~ = E A~ into AIs, x <- conj(A_I~).
+template <int P, int T>
+__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
+  double2 e[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    e[k] = E[T * k];
+    x[k] = As[T * k];
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double2 v = cmul(e[k], x[k]);
+    AIs[T * k] = v;
+    x[k] = conjg(v);
+  }
+}
+// Stage ST of rk4ip_sp once x holds F+(x R(x)) of the stage's field (or,
+// when !fresh, Nt itself): Nt = M F+(.)/N (stored to nA when given); k~ = h
+// Nt (times E at stage 1) enters the bracket S of A'~; x <- conj of the next
+// inverse transform's input. Blocks of C points.
+template <int P, int T, int ST, int C>
+__device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2* nA, double h, double invn,
+                                            const double2* M, const double2* E, double2* S, const double2* AIs,
+                                            double2* out) {
+  static_assert(P % C == 0, "block size");
+#pragma unroll
+  for (int c = 0; c < P; c += C) {
+    double2 m[C], a[C], b[C], e[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      if (fresh) m[j] = M[T * (c + j)];
+      if (ST != 2) e[j] = E[T * (c + j)];
+      if (ST != 1) b[j] = S[T * (c + j)];
+      if (ST == 3) a[j] = AIs[T * (c + j)];
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      double2 n = x[c + j];
+      if (fresh) n = cmul(m[j], cscale(n, invn));
+      if (nA) nA[T * (c + j)] = n;
+      const double2 v = cscale(n, h);
+      double2 w;
+      if (ST == 1) {  // k0~ = E h Nt(A)
+        w = cmul(e[j], v);
+        S[T * (c + j)] = w;
+      } else if (ST == 2) {  // k1~
+        S[T * (c + j)] = cadd(b[j], cscale(v, 2.0));
+        w = v;
+      } else if (ST == 3) {  // k2~: S <- A_I~ + S/6; next input E (A_I~ + k2~)
+        const double2 q = cadd(b[j], cscale(v, 2.0));
+        S[T * (c + j)] = cadd(a[j], make_double2(q.x / 6.0, q.y / 6.0));
+        w = cmul(e[j], cadd(a[j], v));
+      } else {  // k3~: A'~ = k3~/6 + E S
+        w = cadd(make_double2(v.x / 6.0, v.y / 6.0), cmul(e[j], b[j]));
+        if (out) out[T * (c + j)] = w;
+      }
+      x[c + j] = conjg(w);
+    }
+  }
+}
+
 // KIND fixes the nonlinear operator at compile time (NL_KERR, NL_ALL) or
 // leaves it to ops.nl_kind (NL_GENERIC); a fixed kind lets the compiler drop
 // the other operators and their register live ranges.
@@ -113,6 +200,10 @@ struct Dev {
   // that slot addresses are formed at each use instead of being hoisted out
   // of the step loops and kept live for the whole kernel.
   __device__ __forceinline__ unsigned idx(int k) const { return (unsigned)opaque((int)threadIdx.x) + T * k; }
+  // The thread's first owned point of an array; its other points follow at
+  // strides of T (the pointwise helpers below index it so).
+  template <class V>
+  __device__ __forceinline__ V* mine(V* p) const { return p + (unsigned)opaque((int)threadIdx.x); }
   __device__ __forceinline__ int kind() const { return KIND == NL_GENERIC ? op.nl_kind : KIND; }
 
   __device__ __forceinline__ void load(double2 (&x)[P], const double2* __restrict__ g) const {
@@ -392,17 +483,13 @@ struct Dev {
   // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
   __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const { make_exp_of(E, op.lin, h); }
   // exp(factor * h) of a given linear factor (the WDM channels have two)
-  __device__ __forceinline__ void make_exp_of(double2* __restrict__ E, const double2* __restrict__ fac, double h) const {
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-      const double2 f = fac[idx(k)];
-      E[idx(k)] = cexp(make_double2(f.x * h, f.y * h));
-    }
+  __device__ __forceinline__ void make_exp_of(double2* E, const double2* fac, double h) const {
+    pw_exp<P, T>(mine(E), mine(fac), h);
   }
 };
 
 // Per-instance scratch slots (N complex each).
-enum Slot { S_A = 0, S_NA, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, NSLOT };
+enum Slot { S_A = 0, S_NA, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, S_HS, NSLOT };
 
 // solver.py:339-354 rk4ip as a 7-stage loop (one call site per operator),
 // with the linear calls in the reference's order (L(A), L(h N(A)),
@@ -444,21 +531,99 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
     if (st == 0) d.store(sAI, x);         // A_I = L(A)
     else if (st == 2) d.store(sK0, x);    // k0 = L(h N(A))
     else if (st == 3) d.store(sK1, x);    // k1
-    else if (st == 4) {                   // k2 in x
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const int i = d.idx(k);
-        const double2 ai = sAI[i];
-        const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], x[k]), 2.0));
-        sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
-        x[k] = cadd(ai, x[k]);                                  // A_I + k2
-      }
-    } else if (st == 5) d.store(sK1, x);  // k3
+    else if (st == 4) pw_rk4_acc<P, T>(x, d.mine(sAI), d.mine(sK0), d.mine(sK1));  // k2 in x
+    else if (st == 5) d.store(sK1, x);  // k3
     else if (st == 6) {                   // k3/6 + L(acc)
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         const double2 k3 = sK1[d.idx(k)];
         x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
+      }
+    }
+  }
+}
+
+// solver.py:339-354 rk4ip for the full Raman + self-steepening operator
+// (nonlinearity.py:201-207) without gain, carried in the spectral domain
+// between operators. The reference evaluates each linear call as
+// ifft(E fft(x)) and each N(x) as ifft(m fft(x R(x))), m = i gamma (1 +
+// omega ss); wherever a forward transform follows an inverse one the pair
+// cancels, so with Nt(x) = m fft(x R(x)) = fft(N(x)):
+//   A_I~ = E A~,          A_I = ifft(A_I~)
+//   k0~ = E h Nt(A),      k1~ = h Nt(A_I + ifft(k0~)/2)
+//   k2~ = h Nt(A_I + ifft(k1~)/2),  k3~ = h Nt(ifft(E (A_I~ + k2~)))
+//   A'~ = E (A_I~ + (k0~ + 2 (k1~ + k2~))/6) + k3~/6,  A' = ifft(A'~)
+// which is 17 transforms per call instead of 24 (3 fewer when Nt(A) is
+// reused). The transforms run at one call site, pass by pass: p = 0 is the
+// inverse transform to A_I; p = 1..16 are, per stage s = 1..4, the forward
+// transform of |x|^2, the inverse one of the Raman product, the forward one
+// of x R(x) and the inverse one back to the time domain.
+// Input field: time domain in `in`, spectral in `inS`. Result: time domain
+// in x and, when outS is given, spectral in outS (neither of them one of
+// the slots S_AI, S_K0, S_K1, S_U used here).
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
+                                         double2* outS, const double2* E, double h, double2* ws, double2* nA,
+                                         bool nA_ready, bool nA_store) {
+  static_assert(K == NL_ALL && !YB, "spectral-domain rk4ip: full nonlinear operator without gain");
+  using F = FFT<N, T>;
+  constexpr int P = N / T;
+  constexpr double INVN = Dev<N, T, K, YB>::INVN;
+  double2* sAI = ws + S_AI * N;   // A_I, time domain
+  double2* sAIs = ws + S_K0 * N;  // A_I~
+  double2* sS = ws + S_K1 * N;    // k0~, then k0~ + 2 k1~, then the bracket of A'~
+  double2* sU = ws + S_U * N;     // the field x of the running N(x)
+  const Ops& op = d.op;
+  const double a1 = 1.0 - op.f_R;
+  // pass 0 input: conj(A_I~) (the inverse transform is conj . F+ . conj)
+  pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
+  bool skip = false;  // the next pass has no transform (Nt(A) was loaded)
+#pragma unroll 1
+  for (int p = 0; p < 17; ++p) {
+    const int st = (p + 3) >> 2;
+    const int ph = (p + 3) & 3;
+    if (!skip) F::plus(x, d.lds, d.tw);
+    skip = false;
+    if (ph == 0) {  // fft(|x|^2) done: prepare ifft(h_R fft(|x|^2))
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = conjg(cmul(op.hr[d.idx(k)], cscale(x[k], INVN)));
+    } else if (ph == 1) {  // convolution done: x (1-f_R)|x|^2 + f_R x conv
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 a = sU[d.idx(k)];
+        x[k] = cadd(cscale(cscale(a, a1), abs2(a)), cmul(cscale(a, op.f_R), conjg(x[k])));
+      }
+    } else if (ph == 2) {  // fft(x R(x)) done: Nt, then the stage's combination
+      const bool fresh = !(st == 1 && nA_ready);
+      double2* nAo = (st == 1 && nA_store) ? d.mine(nA) : nullptr;
+      const double2* M = d.mine(op.nlmul);
+      constexpr int CW = P < 8 ? P : 8, CN = P < 4 ? P : 4;
+      if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), nullptr, nullptr);
+      else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), nullptr, nullptr);
+      else if (st == 3) pw_sp_stage<P, T, 3, CN>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
+      else pw_sp_stage<P, T, 4, CN>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), nullptr, outS ? d.mine(outS) : nullptr);
+    } else {  // inverse transform done
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = conjg(x[k]);
+      if (st == 4) break;  // A'
+      if (st == 0) {
+        d.store(sAI, x);
+        if (nA_ready) {  // stage 1 starts from the stored Nt(A)
+          d.load(x, nA);
+          p = 2;
+          skip = true;
+          continue;
+        }
+        d.load(x, in);
+      } else if (st < 3) {  // A_I + k/2
+#pragma unroll
+        for (int k = 0; k < P; ++k) x[k] = cadd(sAI[d.idx(k)], cscale(x[k], 0.5));
+      }
+      // next pass: fft(|x|^2), with x parked for the product
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        sU[d.idx(k)] = x[k];
+        x[k] = make_double2(abs2(x[k]), 0.0);
       }
     }
   }
