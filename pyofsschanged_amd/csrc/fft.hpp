@@ -191,6 +191,8 @@ struct FFT {
   struct Twiddles {
     double2 w[NBASE];
   };
+  // tw = exp(+2 pi i m / (S N)): S > 1 reads the table of a longer transform.
+  template <int S = 1>
   __device__ static __forceinline__ void load_twiddles(Twiddles& t, const double2* __restrict__ tw) {
     const int tid = threadIdx.x;
 #pragma unroll
@@ -198,7 +200,7 @@ struct FFT {
       const int ns = plan_ns(N, P, p);
       const int R = plan.radix[p];
 #pragma unroll
-      for (int b = 0; b < nbase(p); ++b) t.w[base_off(p) + b] = tw[((tid + T * b) & (ns - 1)) * (N / (ns * R))];
+      for (int b = 0; b < nbase(p); ++b) t.w[base_off(p) + b] = tw[S * (((tid + T * b) & (ns - 1)) * (N / (ns * R)))];
     }
   }
 

@@ -365,7 +365,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
 template <int N, int T, int METHOD, int K, bool YB, bool SP = false>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
-  __shared__ double lds[FFT<N, T>::LDS_DOUBLES];
+  __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
   __shared__ double red[T / 64 + 2];
   adaptive_body<N, T, METHOD, K, YB, SP>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }

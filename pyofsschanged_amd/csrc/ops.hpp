@@ -90,32 +90,12 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
   return make_double2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// Pointwise passes that load and store different arrays. Each pointer is
-// offset to the calling thread's first owned point and the points follow at
-// strides of T. The compiler keeps loads below earlier stores unless it can
-// prove they differ, which it does not here, so a pass written point by
-// point waits for one memory round trip per point; these passes load a
-// block of C points of every input first, then compute and store it.
-template <int P, int T>
-__device__ __forceinline__ void pw_exp(double2* E, const double2* fac, double h) {
-#pragma unroll
-  for (int k = 0; k < P; ++k) {
-    const double2 f = fac[T * k];
-    E[T * k] = cexp(make_double2(f.x * h, f.y * h));
-  }
-}
-// rk4ip (solver.py:353): acc = A_I + (k0 + 2 (k1 + k2))/6 into K0, and
-// x <- A_I + k2 (x holds k2 on entry).
-template <int P, int T>
-__device__ __forceinline__ void pw_rk4_acc(double2 (&x)[P], const double2* AI, double2* K0, const double2* K1) {
-#pragma unroll
-  for (int k = 0; k < P; ++k) {
-    const double2 ai = AI[T * k];
-    const double2 s = cadd(K0[T * k], cscale(cadd(K1[T * k], x[k]), 2.0));
-    K0[T * k] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));
-    x[k] = cadd(ai, x[k]);
-  }
-}
+// Pointwise passes of rk4ip_sp that load and store different arrays. Each
+// pointer is offset to the calling thread's first owned point and the points
+// follow at strides of T. The compiler keeps loads below earlier stores
+// unless it can prove they differ, which it does not here, so a pass written
+// point by point waits for one memory round trip per point; these passes
+// load a block of C points of every input first, then compute and store it.
 // Entry of rk4ip_sp: A_I~ = E A~ into AIs, x <- conj(A_I~).
 template <int P, int T>
 __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
@@ -419,11 +399,11 @@ struct Dev {
     double2 y[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      sX[idx(k)] = B[k];
       const double2 a = gA[idx(k)];
       const double p = abs2(a);
       y[k] = sel(kind == NL_ALL || kind == NL_RS, make_double2(p, 0.0), cscale(a, p));
     }
+    store(sX, B);
 #pragma unroll 1
     for (int pass = 0; pass < npass; ++pass) {
       int mul;
@@ -483,8 +463,12 @@ struct Dev {
   // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
   __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const { make_exp_of(E, op.lin, h); }
   // exp(factor * h) of a given linear factor (the WDM channels have two)
-  __device__ __forceinline__ void make_exp_of(double2* E, const double2* fac, double h) const {
-    pw_exp<P, T>(mine(E), mine(fac), h);
+  __device__ __forceinline__ void make_exp_of(double2* __restrict__ E, const double2* __restrict__ fac, double h) const {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double2 f = fac[idx(k)];
+      E[idx(k)] = cexp(make_double2(f.x * h, f.y * h));
+    }
   }
 };
 
@@ -531,8 +515,16 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
     if (st == 0) d.store(sAI, x);         // A_I = L(A)
     else if (st == 2) d.store(sK0, x);    // k0 = L(h N(A))
     else if (st == 3) d.store(sK1, x);    // k1
-    else if (st == 4) pw_rk4_acc<P, T>(x, d.mine(sAI), d.mine(sK0), d.mine(sK1));  // k2 in x
-    else if (st == 5) d.store(sK1, x);  // k3
+    else if (st == 4) {                   // k2 in x
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int i = d.idx(k);
+        const double2 ai = sAI[i];
+        const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], x[k]), 2.0));
+        sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+        x[k] = cadd(ai, x[k]);                                  // A_I + k2
+      }
+    } else if (st == 5) d.store(sK1, x);  // k3
     else if (st == 6) {                   // k3/6 + L(acc)
 #pragma unroll
       for (int k = 0; k < P; ++k) {
@@ -541,6 +533,80 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
       }
     }
   }
+}
+
+// LDS (doubles) of a kernel that runs rk4ip_sp: the N-point transform, the
+// N/2-point one and the N real values of the Raman exchanges.
+template <int N, int T>
+constexpr int sp_lds_doubles() {
+  constexpr int a = FFT<N, T>::LDS_DOUBLES, b = FFT<N / 2, T>::LDS_DOUBLES;
+  constexpr int m = a > b ? a : b;
+  return m > N ? m : N;
+}
+
+// The Raman convolution of nonlinearity.py:204, ifft(h_R fft(|x|^2)), of a
+// real signal p = |x|^2 through N/2-point transforms (the standard
+// real-signal method): z_n = p_2n + i p_2n+1 and Z = F+(z) give
+//   P_k = E_k + W^k O_k,  P_k+M = E_k - W^k O_k   (M = N/2, W = e^(2 pi i/N)),
+//   E_k = (Z_k + conj Z_M-k)/2,  O_k = (Z_k - conj Z_M-k)/(2i);
+// with Q = h_R P/N the inverse is y = F-(Q_k + Q_k+M + i (Q_k - Q_k+M) W^-k)
+// over M points, y_m = q_2m + i q_2m+1. The imaginary part of the
+// reference's convolution is rounding of a real quantity (h_R is the
+// transform of a real response) and is dropped. On entry the LDS holds p
+// (raman_stage); on exit x_k holds the convolution at owned point k (real
+// part; imaginary part 0).
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void raman_stage(Dev<N, T, K, YB>& d, const double2 (&x)[N / T]) {
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N / T; ++k) d.lds[d.idx(k)] = abs2(x[k]);
+}
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N / T]) {
+  constexpr int M = N / 2, P = N / T, P2 = M / T;
+  using F2 = FFT<M, T>;
+  constexpr double INVN = 1.0 / N;
+  const Ops& op = d.op;
+  double* ld = d.lds;
+  double2* l2 = reinterpret_cast<double2*>(d.lds);
+  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+  typename F2::Twiddles tw2;
+  F2::template load_twiddles<2>(tw2, op.tw);
+  __syncthreads();
+  double2 z[P2];
+#pragma unroll
+  for (int j = 0; j < P2; ++j) z[j] = l2[t + T * j];
+#pragma unroll 1
+  for (int it = 0; it < 2; ++it) {
+    F2::plus(z, d.lds, tw2);
+    if (it == 0) {  // pair Z_k with Z_M-k, multiply, fold back to M points
+      const unsigned u = (unsigned)opaque((int)threadIdx.x);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < P2; ++j) l2[u + T * j] = z[j];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < P2; ++j) {
+        const double2 w = op.tw[u + T * j];
+        const double2 zr = conjg(l2[(M - (u + T * j)) & (M - 1)]);
+        const double2 e = cscale(cadd(z[j], zr), 0.5);
+        const double2 dz = csub(z[j], zr);
+        const double2 o = make_double2(0.5 * dz.y, -0.5 * dz.x);  // dz / 2i
+        const double2 wo = cmul(w, o);
+        const double2 q0 = cmul(op.hr[u + T * j], cscale(cadd(e, wo), INVN));
+        const double2 q1 = cmul(op.hr[u + T * j + M], cscale(csub(e, wo), INVN));
+        const double2 go = cmul(csub(q0, q1), conjg(w));
+        const double2 y = cadd(cadd(q0, q1), make_double2(-go.y, go.x));  // + i go
+        z[j] = conjg(y);  // F- = conj . F+ . conj
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < P2; ++j) l2[t + T * j] = conjg(z[j]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = make_double2(ld[t + T * k], 0.0);
 }
 
 // solver.py:339-354 rk4ip for the full Raman + self-steepening operator
@@ -553,14 +619,16 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
 //   k0~ = E h Nt(A),      k1~ = h Nt(A_I + ifft(k0~)/2)
 //   k2~ = h Nt(A_I + ifft(k1~)/2),  k3~ = h Nt(ifft(E (A_I~ + k2~)))
 //   A'~ = E (A_I~ + (k0~ + 2 (k1~ + k2~))/6) + k3~/6,  A' = ifft(A'~)
-// which is 17 transforms per call instead of 24 (3 fewer when Nt(A) is
-// reused). The transforms run at one call site, pass by pass: p = 0 is the
-// inverse transform to A_I; p = 1..16 are, per stage s = 1..4, the forward
-// transform of |x|^2, the inverse one of the Raman product, the forward one
-// of x R(x) and the inverse one back to the time domain.
+// which is 9 N-point transforms per call plus 4 Raman convolutions of
+// N/2-point transform pairs (raman_real), instead of the reference's 24
+// N-point transforms. Pass p = 0 is the inverse transform to A_I; passes
+// p = 1..12 are, per stage s = 1..4, the Raman convolution of the stage's
+// field, the forward transform of x R(x) and the inverse one back to the
+// time domain. The N-point transform runs at one call site.
 // Input field: time domain in `in`, spectral in `inS`. Result: time domain
 // in x and, when outS is given, spectral in outS (neither of them one of
-// the slots S_AI, S_K0, S_K1, S_U used here).
+// the slots S_AI, S_K0, S_K1, S_U used here). The kernel's LDS holds
+// sp_lds_doubles<N, T>() doubles.
 template <int N, int T, int K, bool YB>
 __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
                                          double2* outS, const double2* E, double h, double2* ws, double2* nA,
@@ -579,21 +647,21 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
   pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
   bool skip = false;  // the next pass has no transform (Nt(A) was loaded)
 #pragma unroll 1
-  for (int p = 0; p < 17; ++p) {
-    const int st = (p + 3) >> 2;
-    const int ph = (p + 3) & 3;
-    if (!skip) F::plus(x, d.lds, d.tw);
-    skip = false;
-    if (ph == 0) {  // fft(|x|^2) done: prepare ifft(h_R fft(|x|^2))
-#pragma unroll
-      for (int k = 0; k < P; ++k) x[k] = conjg(cmul(op.hr[d.idx(k)], cscale(x[k], INVN)));
-    } else if (ph == 1) {  // convolution done: x (1-f_R)|x|^2 + f_R x conv
+  for (int p = 0; p < 13; ++p) {
+    const int st = (p + 2) / 3;
+    const int ph = (p + 2) % 3;
+    if (ph == 0) {  // x R(x) = (1-f_R) x |x|^2 + f_R x conv, x parked in sU
+      raman_real(d, x);
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         const double2 a = sU[d.idx(k)];
-        x[k] = cadd(cscale(cscale(a, a1), abs2(a)), cmul(cscale(a, op.f_R), conjg(x[k])));
+        x[k] = cadd(cscale(cscale(a, a1), abs2(a)), cscale(cscale(a, op.f_R), x[k].x));
       }
-    } else if (ph == 2) {  // fft(x R(x)) done: Nt, then the stage's combination
+      continue;
+    }
+    if (!skip) F::plus(x, d.lds, d.tw);
+    skip = false;
+    if (ph == 1) {  // fft(x R(x)) done: Nt, then the stage's combination
       const bool fresh = !(st == 1 && nA_ready);
       double2* nAo = (st == 1 && nA_store) ? d.mine(nA) : nullptr;
       const double2* M = d.mine(op.nlmul);
@@ -610,7 +678,7 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
         d.store(sAI, x);
         if (nA_ready) {  // stage 1 starts from the stored Nt(A)
           d.load(x, nA);
-          p = 2;
+          p = 1;
           skip = true;
           continue;
         }
@@ -619,12 +687,8 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
 #pragma unroll
         for (int k = 0; k < P; ++k) x[k] = cadd(sAI[d.idx(k)], cscale(x[k], 0.5));
       }
-      // next pass: fft(|x|^2), with x parked for the product
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        sU[d.idx(k)] = x[k];
-        x[k] = make_double2(abs2(x[k]), 0.0);
-      }
+      d.store(sU, x);  // parked for the product after the convolution
+      raman_stage(d, x);
     }
   }
 }
