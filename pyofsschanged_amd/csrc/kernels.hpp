@@ -254,8 +254,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         h = prm.replay_h[n_att];
       }
       const double h_half = 0.5 * h;
-      d.make_exp(sEH, 0.5 * h);
-      d.make_exp(sEQ, 0.5 * h_half);
+      if constexpr (SP) {
+        d.make_exp_pair(sEQ, sEH, 0.5 * h_half);
+      } else {
+        d.make_exp(sEH, 0.5 * h);
+        d.make_exp(sEQ, 0.5 * h_half);
+      }
       // stepper.py:303-308 in its order (the Yb pump state depends on it):
       // half step, fine step (both h/2), then the coarse step h. The fine
       // result replaces the half-step field in sH; the coarse one stays in x.

@@ -96,7 +96,7 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
 // unless it can prove they differ, which it does not here, so a pass written
 // point by point waits for one memory round trip per point; these passes
 // load a block of C points of every input first, then compute and store it.
-// Entry of rk4ip_sp: A_I~ = E A~ into AIs, x <- conj(A_I~).
+// Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer).
 template <int P, int T>
 __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
   double2 e[P];
@@ -106,16 +106,13 @@ __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, c
     x[k] = As[T * k];
   }
 #pragma unroll
-  for (int k = 0; k < P; ++k) {
-    const double2 v = cmul(e[k], x[k]);
-    AIs[T * k] = v;
-    x[k] = conjg(v);
-  }
+  for (int k = 0; k < P; ++k) AIs[T * k] = cmul(e[k], x[k]);
 }
 // Stage ST of rk4ip_sp once x holds F+(x R(x)) of the stage's field (or,
 // when !fresh, Nt itself): Nt = M F+(.)/N (stored to nA when given); k~ = h
-// Nt (times E at stage 1) enters the bracket S of A'~; x <- conj of the next
-// inverse transform's input. Blocks of C points.
+// Nt (times E at stage 1) enters the bracket S of A'~; x <- conj of the
+// spectrum of the next stage's field (A_I~ + k~/2, E (A_I~ + k2~), or A'~).
+// Blocks of C points.
 template <int P, int T, int ST, int C>
 __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2* nA, double h, double invn,
                                             const double2* M, const double2* E, double2* S, const double2* AIs,
@@ -129,7 +126,7 @@ __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2
       if (fresh) m[j] = M[T * (c + j)];
       if (ST != 2) e[j] = E[T * (c + j)];
       if (ST != 1) b[j] = S[T * (c + j)];
-      if (ST == 3) a[j] = AIs[T * (c + j)];
+      if (ST != 4) a[j] = AIs[T * (c + j)];
     }
 #pragma unroll
     for (int j = 0; j < C; ++j) {
@@ -138,12 +135,13 @@ __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2
       if (nA) nA[T * (c + j)] = n;
       const double2 v = cscale(n, h);
       double2 w;
-      if (ST == 1) {  // k0~ = E h Nt(A)
-        w = cmul(e[j], v);
-        S[T * (c + j)] = w;
-      } else if (ST == 2) {  // k1~
+      if (ST == 1) {  // k0~ = E h Nt(A); next input A_I~ + k0~/2
+        const double2 k0 = cmul(e[j], v);
+        S[T * (c + j)] = k0;
+        w = cadd(a[j], cscale(k0, 0.5));
+      } else if (ST == 2) {  // k1~; next input A_I~ + k1~/2
         S[T * (c + j)] = cadd(b[j], cscale(v, 2.0));
-        w = v;
+        w = cadd(a[j], cscale(v, 0.5));
       } else if (ST == 3) {  // k2~: S <- A_I~ + S/6; next input E (A_I~ + k2~)
         const double2 q = cadd(b[j], cscale(v, 2.0));
         S[T * (c + j)] = cadd(a[j], make_double2(q.x / 6.0, q.y / 6.0));
@@ -462,6 +460,17 @@ struct Dev {
 
   // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
   __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const { make_exp_of(E, op.lin, h); }
+  // E1 <- exp(lin * h) and E2 <- E1^2 = exp(lin * 2h) (to rounding: one
+  // complex exponential per point for the two step sizes of an attempt)
+  __device__ __forceinline__ void make_exp_pair(double2* __restrict__ E1, double2* __restrict__ E2, double h) const {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double2 f = op.lin[idx(k)];
+      const double2 e = cexp(make_double2(f.x * h, f.y * h));
+      E1[idx(k)] = e;
+      E2[idx(k)] = cmul(e, e);
+    }
+  }
   // exp(factor * h) of a given linear factor (the WDM channels have two)
   __device__ __forceinline__ void make_exp_of(double2* __restrict__ E, const double2* __restrict__ fac, double h) const {
 #pragma unroll
@@ -615,19 +624,20 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 // ifft(E fft(x)) and each N(x) as ifft(m fft(x R(x))), m = i gamma (1 +
 // omega ss); wherever a forward transform follows an inverse one the pair
 // cancels, so with Nt(x) = m fft(x R(x)) = fft(N(x)):
-//   A_I~ = E A~,          A_I = ifft(A_I~)
-//   k0~ = E h Nt(A),      k1~ = h Nt(A_I + ifft(k0~)/2)
-//   k2~ = h Nt(A_I + ifft(k1~)/2),  k3~ = h Nt(ifft(E (A_I~ + k2~)))
+//   A_I~ = E A~,          k0~ = E h Nt(A),
+//   k1~ = h Nt(ifft(A_I~ + k0~/2)),  k2~ = h Nt(ifft(A_I~ + k1~/2)),
+//   k3~ = h Nt(ifft(E (A_I~ + k2~))),
 //   A'~ = E (A_I~ + (k0~ + 2 (k1~ + k2~))/6) + k3~/6,  A' = ifft(A'~)
-// which is 9 N-point transforms per call plus 4 Raman convolutions of
-// N/2-point transform pairs (raman_real), instead of the reference's 24
-// N-point transforms. Pass p = 0 is the inverse transform to A_I; passes
-// p = 1..12 are, per stage s = 1..4, the Raman convolution of the stage's
-// field, the forward transform of x R(x) and the inverse one back to the
-// time domain. The N-point transform runs at one call site.
+// (the reference forms A_I + k/2 in the time domain; the sum of spectra is
+// the same field). That is 8 N-point transforms per call plus 4 Raman
+// convolutions of N/2-point transform pairs (raman_real), instead of the
+// reference's 24 N-point transforms. Passes p = 0..11 are, per stage
+// s = 1..4, the Raman convolution of the stage's field, the forward
+// transform of x R(x) and the inverse transform of the next stage's field;
+// the N-point transform runs at one call site.
 // Input field: time domain in `in`, spectral in `inS`. Result: time domain
 // in x and, when outS is given, spectral in outS (neither of them one of
-// the slots S_AI, S_K0, S_K1, S_U used here). The kernel's LDS holds
+// the slots S_K0, S_K1, S_U used here). The kernel's LDS holds
 // sp_lds_doubles<N, T>() doubles.
 template <int N, int T, int K, bool YB>
 __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
@@ -637,24 +647,31 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
   using F = FFT<N, T>;
   constexpr int P = N / T;
   constexpr double INVN = Dev<N, T, K, YB>::INVN;
-  double2* sAI = ws + S_AI * N;   // A_I, time domain
   double2* sAIs = ws + S_K0 * N;  // A_I~
   double2* sS = ws + S_K1 * N;    // k0~, then k0~ + 2 k1~, then the bracket of A'~
   double2* sU = ws + S_U * N;     // the field x of the running N(x)
   const Ops& op = d.op;
   const double a1 = 1.0 - op.f_R;
-  // pass 0 input: conj(A_I~) (the inverse transform is conj . F+ . conj)
   pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
-  bool skip = false;  // the next pass has no transform (Nt(A) was loaded)
+  int p0 = 0;
+  if (nA_ready) {  // stage 1 starts from the stored Nt(A)
+    d.load(x, nA);
+    p0 = 1;
+  } else {  // the product after the convolution reads A from `in`
+    d.load(x, in);
+    raman_stage(d, x);
+  }
+  bool skip = nA_ready;  // the next pass has no transform (Nt(A) was loaded)
 #pragma unroll 1
-  for (int p = 0; p < 13; ++p) {
-    const int st = (p + 2) / 3;
-    const int ph = (p + 2) % 3;
+  for (int p = p0; p < 12; ++p) {
+    const int st = p / 3 + 1;
+    const int ph = p % 3;
     if (ph == 0) {  // x R(x) = (1-f_R) x |x|^2 + f_R x conv, x parked in sU
       raman_real(d, x);
+      const double2* xs = st == 1 ? in : sU;
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        const double2 a = sU[d.idx(k)];
+        const double2 a = xs[d.idx(k)];
         x[k] = cadd(cscale(cscale(a, a1), abs2(a)), cscale(cscale(a, op.f_R), x[k].x));
       }
       continue;
@@ -665,28 +682,15 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
       const bool fresh = !(st == 1 && nA_ready);
       double2* nAo = (st == 1 && nA_store) ? d.mine(nA) : nullptr;
       const double2* M = d.mine(op.nlmul);
-      constexpr int CW = P < 8 ? P : 8, CN = P < 4 ? P : 4;
-      if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), nullptr, nullptr);
-      else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), nullptr, nullptr);
-      else if (st == 3) pw_sp_stage<P, T, 3, CN>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
-      else pw_sp_stage<P, T, 4, CN>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), nullptr, outS ? d.mine(outS) : nullptr);
-    } else {  // inverse transform done
+      constexpr int CW = P < 4 ? P : 4;
+      if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
+      else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
+      else if (st == 3) pw_sp_stage<P, T, 3, CW>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
+      else pw_sp_stage<P, T, 4, CW>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), nullptr, outS ? d.mine(outS) : nullptr);
+    } else {  // inverse transform done: the next stage's field (or A')
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = conjg(x[k]);
-      if (st == 4) break;  // A'
-      if (st == 0) {
-        d.store(sAI, x);
-        if (nA_ready) {  // stage 1 starts from the stored Nt(A)
-          d.load(x, nA);
-          p = 1;
-          skip = true;
-          continue;
-        }
-        d.load(x, in);
-      } else if (st < 3) {  // A_I + k/2
-#pragma unroll
-        for (int k = 0; k < P; ++k) x[k] = cadd(sAI[d.idx(k)], cscale(x[k], 0.5));
-      }
+      if (st == 4) break;
       d.store(sU, x);  // parked for the product after the convolution
       raman_stage(d, x);
     }
