@@ -652,14 +652,15 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
   double2* sU = ws + S_U * N;     // the field x of the running N(x)
   const Ops& op = d.op;
   const double a1 = 1.0 - op.f_R;
-  pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
   int p0 = 0;
   if (nA_ready) {  // stage 1 starts from the stored Nt(A)
+    pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
     d.load(x, nA);
     p0 = 1;
   } else {  // the product after the convolution reads A from `in`
     d.load(x, in);
     raman_stage(d, x);
+    pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
   }
   bool skip = nA_ready;  // the next pass has no transform (Nt(A) was loaded)
 #pragma unroll 1
