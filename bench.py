@@ -366,7 +366,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "accounting": ("achieved counts the reference's fft_counter bytes, 72 N-point FFTs x 32 N B per "
-                           "attempted step; the kernel runs 27 N-point transforms and 11 Raman convolutions "
+                           "attempted step; the kernel runs 24 N-point transforms and 11 Raman convolutions "
                            "through N/2-point transform pairs per attempted step, none of them through HBM, so "
                            "frac can exceed 1; traffic is the measured HBM bytes of the launch"),
         },
