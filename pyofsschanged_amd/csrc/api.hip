@@ -244,6 +244,25 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
   c->ops.gain_log = gain_log;
   c->ops.gain_len2 = 2.0 * length;
   c->ops.esat = E_sat;
+  // the old Simpson rule of an even number of points weighs every interior
+  // point alike, up to the rounding of the grid spacing (1.1e-12 relative on
+  // a 16384-point grid): then the spectral-domain stepper forms energies from
+  // spectra (kernels.hpp, k_adaptive with SP = 2), with a relative error
+  // below the tolerance accepted here
+  c->ops.simp_flat = 0;
+  if (n >= 8) {
+    const double w = simpson_w[2];
+    bool flat = w > 0.0;
+    for (size_t i = 2; i + 2 < n && flat; ++i) flat = fabs(simpson_w[i] - w) <= 1e-10 * w;
+    if (flat) {
+      c->ops.simp_flat = 1;
+      c->ops.simp_c = w;
+      c->ops.simp_d[0] = simpson_w[0] - w;
+      c->ops.simp_d[1] = simpson_w[1] - w;
+      c->ops.simp_d[2] = simpson_w[n - 2] - w;
+      c->ops.simp_d[3] = simpson_w[n - 1] - w;
+    }
+  }
   refresh_ops(c);
   return GNLSE_OK;
 }

@@ -193,10 +193,10 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // instance b of a batch whose inputs, outputs and workspaces start at in,
 // out and ws_all. With spec != nullptr the pyofss spectrum fft(A_out) is
 // also written to spec + b*N (input of the ensemble coherence).
-// SP selects rk4ip_sp (RK4IP, full nonlinear operator, no gain): the
-// accepted field is then also kept as its spectrum in S_T and the half
-// step's spectrum, left in S_HS, starts the fine step.
-template <int N, int T, int METHOD, int K, bool YB = false, bool SP = false>
+// SP selects rk4ip_sp (RK4IP, full nonlinear operator; 1: no gain, 2: the
+// saturable gain): the accepted field is then also kept as its spectrum in
+// S_T and the half step's spectrum, left in S_HS, starts the fine step.
+template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
                                               double2* ws_all, double2* __restrict__ spec) {
@@ -237,7 +237,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   };
 
   if (prm.traces != 1 && prm.max_traces > 0) store_trace(0.0, 0.0);  // stepper.py:284-285
-  if constexpr (SP) {
+  if constexpr (SP != 0) {
     d.fft(x);
     d.store(sAs, x);
   }
@@ -254,7 +254,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         h = prm.replay_h[n_att];
       }
       const double h_half = 0.5 * h;
-      if constexpr (SP) {
+      if constexpr (SP != 0) {
         d.make_exp_pair(sEQ, sEH, 0.5 * h_half);
       } else {
         d.make_exp(sEH, 0.5 * h);
@@ -270,9 +270,9 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         // N(A) is shared by the half and the coarse step, and by every
         // attempt from the same accepted field
         const bool from_a = (c != 1);
-        if constexpr (SP)
-          rk4ip_sp(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
-                   from_a && nA_valid, from_a && !nA_valid);
+        if constexpr (SP != 0)
+          rk4ip_sp<SP == 2>(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
+                            from_a && nA_valid, from_a && !nA_valid);
         else
           integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
         if (from_a) nA_valid = true;
@@ -324,7 +324,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
           if (prm.max_traces > 0 && !store_trace(z, h_temp)) status = GNLSE_STATUS_TRACE_OVERFLOW;
           z_ignore += prm.step_storage;
         }
-        if constexpr (SP) {
+        if constexpr (SP != 0) {
           d.fft(x);
           d.store(sAs, x);
         }
@@ -366,11 +366,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
 }
 
-template <int N, int T, int METHOD, int K, bool YB, bool SP = false>
+template <int N, int T, int METHOD, int K, bool YB, int SP = 0>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
   __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
-  __shared__ double red[T / 64 + 2];
+  __shared__ double red[SP == 2 ? 18 * (T / 64) + 2 : T / 64 + 2];
   adaptive_body<N, T, METHOD, K, YB, SP>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
@@ -462,7 +462,9 @@ struct Launcher {
     else if (o.wdm)
       hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.gain == GAIN_NONE && o.nl_kind == NL_ALL)
-      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, true>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.gain == GAIN_SATURABLE && o.simp_flat && o.nl_kind == NL_ALL)
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 2>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP)
       GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);

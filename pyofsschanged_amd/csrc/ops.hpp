@@ -43,6 +43,11 @@ struct Ops {
   double esat;              // saturation energy
   const double* gprof;      // spectral gain profile, FFT order
   const double* simpw;      // Simpson weights of the energy integral
+  // simpw = simp_c everywhere but at points 0, 1, n-2, n-1, where it is
+  // simp_c + simp_d[0..3] (simp_flat), so the energy of a field follows from
+  // its spectrum (Parseval) and four time samples
+  int simp_flat;
+  double simp_c, simp_d[4];
   // two-level Yb model (amplifier.py:178-402), arrays in FFT order; terms of
   // zero cross section (NaN or inf in the reference) hold 0
   const double* yb_rs;      // ratio_s / Psat_s
@@ -96,9 +101,46 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
 // unless it can prove they differ, which it does not here, so a pass written
 // point by point waits for one memory round trip per point; these passes
 // load a block of C points of every input first, then compute and store it.
-// Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer).
-template <int P, int T>
-__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
+// Energy of a field (field.py:161-172, the Simpson sum of |A|^2 over t) from
+// its spectrum X = fft(A) when the weights are simp_c except at points 0,
+// 1, n-2, n-1 (Ops::simp_flat): E = simp_c n sum |X_k|^2 + sum_j simp_d[j]
+// |A_j|^2, with A_j = sum_k X_k exp(-2 pi i k j / n). energy_acc adds point
+// i's terms to the partial sums a = [sum |X|^2, A_0, A_1, A_n-1, A_n-2]
+// (complex samples as two doubles); energy_of forms E from the reduced sums.
+template <int N>
+__device__ __forceinline__ void energy_acc(double* a, double2 X, unsigned i, const double2* __restrict__ tw) {
+  a[0] += abs2(X);
+  a[1] += X.x;
+  a[2] += X.y;
+  const double2 w1 = tw[i];
+  const double2 p1 = cmul(X, conjg(w1));
+  a[3] += p1.x;
+  a[4] += p1.y;
+  const double2 m1 = cmul(X, w1);
+  a[5] += m1.x;
+  a[6] += m1.y;
+  const double2 m2 = cmul(X, tw[(2u * i) & (unsigned)(N - 1)]);
+  a[7] += m2.x;
+  a[8] += m2.y;
+}
+template <int N>
+__device__ __forceinline__ double energy_of(const Ops& op, const double* a) {
+  return op.simp_c * ((double)N * a[0]) + op.simp_d[0] * (a[1] * a[1] + a[2] * a[2]) +
+         op.simp_d[1] * (a[3] * a[3] + a[4] * a[4]) + op.simp_d[2] * (a[7] * a[7] + a[8] * a[8]) +
+         op.simp_d[3] * (a[5] * a[5] + a[6] * a[6]);
+}
+// amplifier.py:149-158: the gain exponent g of a linear call of step hl on a
+// field of energy En (as Dev::linear forms it).
+__device__ __forceinline__ double sat_gain(const Ops& op, double En, double hl) {
+  const double G0 = (op.gain_log * hl) / op.gain_len2;
+  return G0 / (1.0 + En / op.esat);
+}
+
+// Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer); with the
+// saturable gain (SG) times exp(g profile).
+template <int P, int T, bool SG = false>
+__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs,
+                                            double g = 0.0, const double* prof = nullptr) {
   double2 e[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -106,7 +148,10 @@ __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, c
     x[k] = As[T * k];
   }
 #pragma unroll
-  for (int k = 0; k < P; ++k) AIs[T * k] = cmul(e[k], x[k]);
+  for (int k = 0; k < P; ++k) {
+    if constexpr (SG) AIs[T * k] = cscale(cmul(e[k], x[k]), exp(g * prof[T * k]));
+    else AIs[T * k] = cmul(e[k], x[k]);
+  }
 }
 // Stage ST of rk4ip_sp once x holds F+(x R(x)) of the stage's field (or,
 // when !fresh, Nt itself): Nt = M F+(.)/N (stored to nA when given); k~ = h
@@ -266,6 +311,30 @@ struct Dev {
 #pragma unroll
     for (int w = 0; w < NW; ++w) s += red[w];
     return s;
+  }
+
+  // M block sums at once through buf (M * NW doubles of LDS, not in use);
+  // each value is reduced in the order of block_sum.
+  template <int M>
+  __device__ __forceinline__ void block_sum_n(double (&v)[M], double* buf) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m] += __shfl_xor(v[m], o, 64);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) buf[m * NW + (threadIdx.x >> 6)] = v[m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) s += buf[m * NW + w];
+      v[m] = s;
+    }
   }
 
   // Pump power of instance b (the Yb model only): read before the first
@@ -544,6 +613,107 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
   }
 }
 
+// The stages of rk4ip_sp whose linear call carries the saturable gain
+// (amplifier.py:149-158): the gain of L(y) follows from the energy of y, so
+// the spectrum y is formed and its energy reduced over the workgroup before
+// it is multiplied by exp(g profile) E.
+//   ST = 1: y = h Nt(A) (x holds F+(x R(x)), or Nt(A) when !fresh);
+//           k0~ = exp(g y) E y into S, x <- conj(A_I~ + k0~/2).
+//   ST = 3: k2~ = h Nt; S <- A_I~ + (S + 2 k2~)/6; y = A_I~ + k2~;
+//           x <- conj(exp(g(y)) E y); g4 <- the gain exponent of S (L(acc)).
+//   ST = 4: x <- conj(k3~/6 + exp(g4) E S), A'~ also to out.
+// red holds 18 NW doubles.
+template <int N, int T, int ST, int C, int KIND, bool YB>
+__device__ __forceinline__ void pw_sp_stage_sg(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], bool fresh, double2* nA,
+                                               double h, double hl, const double2* E, double2* S, const double2* AIs,
+                                               double2* out, double& g4, double* red) {
+  constexpr int P = N / T;
+  constexpr double INVN = 1.0 / N;
+  const Ops& op = d.op;
+  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+  const double* prof = d.mine(op.gprof);
+  if constexpr (ST == 1) {
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      double2 n = x[k];
+      if (fresh) n = cmul(op.nlmul[t + T * k], cscale(n, INVN));
+      if (nA) nA[t + T * k] = n;
+      x[k] = cscale(n, h);
+      energy_acc<N>(a, x[k], t + T * k, op.tw);
+    }
+    d.block_sum_n(a, red);
+    const double g = sat_gain(op, energy_of<N>(op, a), hl);
+#pragma unroll
+    for (int c = 0; c < P; c += C) {
+      double2 e[C], ai[C];
+      double pr[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        e[j] = E[T * (c + j)];
+        ai[j] = AIs[T * (c + j)];
+        pr[j] = prof[T * (c + j)];
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const double2 k0 = cscale(cmul(e[j], x[c + j]), exp(g * pr[j]));
+        S[T * (c + j)] = k0;
+        x[c + j] = conjg(cadd(ai[j], cscale(k0, 0.5)));
+      }
+    }
+  } else if constexpr (ST == 3) {
+    double a[18];
+#pragma unroll
+    for (int m = 0; m < 18; ++m) a[m] = 0.0;
+#pragma unroll
+    for (int c = 0; c < P; c += C) {
+      double2 m[C], ai[C], b[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        m[j] = op.nlmul[t + T * (c + j)];
+        ai[j] = AIs[T * (c + j)];
+        b[j] = S[T * (c + j)];
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const double2 v = cscale(cmul(m[j], cscale(x[c + j], INVN)), h);
+        const double2 q = cadd(b[j], cscale(v, 2.0));
+        const double2 sn = cadd(ai[j], make_double2(q.x / 6.0, q.y / 6.0));
+        S[T * (c + j)] = sn;
+        x[c + j] = cadd(ai[j], v);
+        energy_acc<N>(a, x[c + j], t + T * (c + j), op.tw);
+        energy_acc<N>(a + 9, sn, t + T * (c + j), op.tw);
+      }
+    }
+    d.block_sum_n(a, red);
+    const double g = sat_gain(op, energy_of<N>(op, a), hl);
+    g4 = sat_gain(op, energy_of<N>(op, a + 9), hl);
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = conjg(cscale(cmul(E[T * k], x[k]), exp(g * prof[T * k])));
+  } else {
+    static_assert(ST == 4, "stages 1, 3 and 4 carry a linear call");
+#pragma unroll
+    for (int c = 0; c < P; c += C) {
+      double2 m[C], b[C], e[C];
+      double pr[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        m[j] = op.nlmul[t + T * (c + j)];
+        b[j] = S[T * (c + j)];
+        e[j] = E[T * (c + j)];
+        pr[j] = prof[T * (c + j)];
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        const double2 v = cscale(cmul(m[j], cscale(x[c + j], INVN)), h);
+        const double2 w = cadd(make_double2(v.x / 6.0, v.y / 6.0), cscale(cmul(e[j], b[j]), exp(g4 * pr[j])));
+        if (out) out[T * (c + j)] = w;
+        x[c + j] = conjg(w);
+      }
+    }
+  }
+}
+
 // LDS (doubles) of a kernel that runs rk4ip_sp: the N-point transform, the
 // N/2-point one and the N real values of the Raman exchanges.
 template <int N, int T>
@@ -639,11 +809,15 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 // in x and, when outS is given, spectral in outS (neither of them one of
 // the slots S_K0, S_K1, S_U used here). The kernel's LDS holds
 // sp_lds_doubles<N, T>() doubles.
-template <int N, int T, int K, bool YB>
+// SG: the saturable gain of amplifier.py:149-158 in every linear call (the
+// energies of the arrays the reference passes to them: A from its time
+// samples, the others from their spectra; Ops::simp_flat must hold; the
+// workgroup's red buffer holds 18 NW doubles).
+template <bool SG, int N, int T, int K, bool YB>
 __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
                                          double2* outS, const double2* E, double h, double2* ws, double2* nA,
                                          bool nA_ready, bool nA_store) {
-  static_assert(K == NL_ALL && !YB, "spectral-domain rk4ip: full nonlinear operator without gain");
+  static_assert(K == NL_ALL && !YB, "spectral-domain rk4ip: full nonlinear operator, no Yb gain");
   using F = FFT<N, T>;
   constexpr int P = N / T;
   constexpr double INVN = Dev<N, T, K, YB>::INVN;
@@ -653,7 +827,20 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
   const Ops& op = d.op;
   const double a1 = 1.0 - op.f_R;
   int p0 = 0;
-  if (nA_ready) {  // stage 1 starts from the stored Nt(A)
+  double g4 = 0.0;  // SG: gain exponent of L(acc), from stage 3
+  if constexpr (SG) {  // L(A): energy of A from its time samples (field.py:161-172)
+    d.load(x, in);
+    double e = 0.0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) e += op.simpw[d.idx(k)] * abs2(x[k]);
+    const double g0 = sat_gain(op, d.block_sum(e), 0.5 * h);
+    if (!nA_ready) raman_stage(d, x);
+    pw_sp_entry<P, T, true>(x, d.mine(E), d.mine(inS), d.mine(sAIs), g0, d.mine(op.gprof));
+    if (nA_ready) {
+      d.load(x, nA);
+      p0 = 1;
+    }
+  } else if (nA_ready) {  // stage 1 starts from the stored Nt(A)
     pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
     d.load(x, nA);
     p0 = 1;
@@ -684,6 +871,18 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
       double2* nAo = (st == 1 && nA_store) ? d.mine(nA) : nullptr;
       const double2* M = d.mine(op.nlmul);
       constexpr int CW = P < 4 ? P : 4;
+      if constexpr (SG) {
+        const double hl = 0.5 * h;
+        double2* out = outS ? d.mine(outS) : nullptr;
+        if (st == 1)
+          pw_sp_stage_sg<N, T, 1, CW>(d, x, fresh, (st == 1 && nA_store) ? nA : nullptr, h, hl, d.mine(E), d.mine(sS),
+                                      d.mine(sAIs), nullptr, g4, d.red);
+        else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
+        else if (st == 3)
+          pw_sp_stage_sg<N, T, 3, CW>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr, g4, d.red);
+        else pw_sp_stage_sg<N, T, 4, CW>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), nullptr, out, g4, d.red);
+        continue;
+      }
       if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
       else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
       else if (st == 3) pw_sp_stage<P, T, 3, CW>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
