@@ -160,22 +160,34 @@ __global__ void __launch_bounds__(T)
   d.store(out + b * N, x);
 }
 
-// stepper.py:164-222 with the cached exp(factor*h/2) in ops.efix
-template <int N, int T, int METHOD, int K, bool YB>
+// stepper.py:164-222 with the cached exp(factor*h/2) in ops.efix. SP as for
+// k_adaptive: rk4ip_sp with gain SP - 1, the field's spectrum carried from
+// step to step in S_T.
+template <int N, int T, int METHOD, int K, bool YB, int SP = 0>
 __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const double2* __restrict__ in,
                                              double2* __restrict__ out, double2* ws_all) {
-  GN_SHARED_G(N, T, K, YB);
+  static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
+                "spectral-domain stepping: rk4ip, NL_ALL");
+  __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
+  __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
+  Dev<N, T, K, YB> d(lds, red, ops);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
   double2* sA = ws + S_A * N;
+  double2* sAs = ws + S_T * N;
   double2 x[P];
   d.load(x, in + b * N);
   d.store(sA, x);
+  if constexpr (SP != 0) {
+    d.fft(x);
+    d.store(sAs, x);
+  }
   d.pump_begin(b);
   long long n_tr = 0;
   for (long long i = 0; i < prm.nsteps; ++i) {
-    integrator<METHOD>(d, x, sA, ops.efix, prm.h, ws);
+    if constexpr (SP != 0) rk4ip_sp<SP - 1>(d, x, sA, sAs, sAs, ops.efix, prm.h, ws, nullptr, false, false);
+    else integrator<METHOD>(d, x, sA, ops.efix, prm.h, ws);
     d.store(sA, x);
     if (prm.trace_every > 0 && (i % prm.trace_every) == 0) {  // stepper.py:206-209
       if (n_tr < prm.max_traces) d.store(prm.tr_A + (b * prm.max_traces + n_tr) * N, x);
@@ -193,14 +205,16 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // instance b of a batch whose inputs, outputs and workspaces start at in,
 // out and ws_all. With spec != nullptr the pyofss spectrum fft(A_out) is
 // also written to spec + b*N (input of the ensemble coherence).
-// SP selects rk4ip_sp (RK4IP, full nonlinear operator; 1: no gain, 2: the
-// saturable gain): the accepted field is then also kept as its spectrum in
-// S_T and the half step's spectrum, left in S_HS, starts the fine step.
+// SP selects rk4ip_sp (RK4IP, full nonlinear operator) with gain SP - 1
+// (SpGain: 1 none, 2 saturable, 3 two-level Yb): the accepted field is then
+// also kept as its spectrum in S_T and the half step's spectrum, left in
+// S_HS, starts the fine step.
 template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
                                               double2* ws_all, double2* __restrict__ spec) {
-  static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && !YB), "spectral-domain stepping: rk4ip, NL_ALL, no gain");
+  static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
+                "spectral-domain stepping: rk4ip, NL_ALL");
   Dev<N, T, K, YB> d(lds, red, ops);
   constexpr int P = N / T;
   double2* ws = ws_all + b * NSLOT * N;
@@ -271,8 +285,8 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         // attempt from the same accepted field
         const bool from_a = (c != 1);
         if constexpr (SP != 0)
-          rk4ip_sp<SP == 2>(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
-                            from_a && nA_valid, from_a && !nA_valid);
+          rk4ip_sp<SP - 1>(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
+                           from_a && nA_valid, from_a && !nA_valid);
         else
           integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
         if (from_a) nA_valid = true;
@@ -370,7 +384,7 @@ template <int N, int T, int METHOD, int K, bool YB, int SP = 0>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
   __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
-  __shared__ double red[SP == 2 ? 18 * (T / 64) + 2 : T / 64 + 2];
+  __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
   adaptive_body<N, T, METHOD, K, YB, SP>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
@@ -451,6 +465,10 @@ struct Launcher {
       hipLaunchKernelGGL((k_fixed_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.wdm)
       hipLaunchKernelGGL((k_fixed_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_YB)
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_NONE)
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();

@@ -136,11 +136,9 @@ __device__ __forceinline__ double sat_gain(const Ops& op, double En, double hl) 
   return G0 / (1.0 + En / op.esat);
 }
 
-// Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer); with the
-// saturable gain (SG) times exp(g profile).
-template <int P, int T, bool SG = false>
-__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs,
-                                            double g = 0.0, const double* prof = nullptr) {
+// Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer).
+template <int P, int T>
+__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
   double2 e[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -148,10 +146,7 @@ __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, c
     x[k] = As[T * k];
   }
 #pragma unroll
-  for (int k = 0; k < P; ++k) {
-    if constexpr (SG) AIs[T * k] = cscale(cmul(e[k], x[k]), exp(g * prof[T * k]));
-    else AIs[T * k] = cmul(e[k], x[k]);
-  }
+  for (int k = 0; k < P; ++k) AIs[T * k] = cmul(e[k], x[k]);
 }
 // Stage ST of rk4ip_sp once x holds F+(x R(x)) of the stage's field (or,
 // when !fresh, Nt itself): Nt = M F+(.)/N (stored to nA when given); k~ = h
@@ -613,58 +608,139 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], 
   }
 }
 
-// The stages of rk4ip_sp whose linear call carries the saturable gain
-// (amplifier.py:149-158): the gain of L(y) follows from the energy of y, so
-// the spectrum y is formed and its energy reduced over the workgroup before
-// it is multiplied by exp(g profile) E.
-//   ST = 1: y = h Nt(A) (x holds F+(x R(x)), or Nt(A) when !fresh);
-//           k0~ = exp(g y) E y into S, x <- conj(A_I~ + k0~/2).
-//   ST = 3: k2~ = h Nt; S <- A_I~ + (S + 2 k2~)/6; y = A_I~ + k2~;
-//           x <- conj(exp(g(y)) E y); g4 <- the gain exponent of S (L(acc)).
-//   ST = 4: x <- conj(k3~/6 + exp(g4) E S), A'~ also to out.
-// red holds 18 NW doubles.
-template <int N, int T, int ST, int C, int KIND, bool YB>
-__device__ __forceinline__ void pw_sp_stage_sg(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], bool fresh, double2* nA,
-                                               double h, double hl, const double2* E, double2* S, const double2* AIs,
-                                               double2* out, double& g4, double* red) {
+// Gain models of rk4ip_sp's linear calls.
+enum SpGain { SPG_NONE = 0, SPG_SAT = 1, SPG_YB = 2 };
+// Partial sums that fix a linear call's gain, per point of the spectrum Y
+// of the array passed to it: the energy terms (SPG_SAT, 9 sums) or the
+// two spectral-power sums of amplifier.py:376-381 (SPG_YB).
+template <int GK>
+struct SpSums {
+  static constexpr int M = GK == SPG_SAT ? 9 : 2;
+};
+template <int N, int GK>
+__device__ __forceinline__ void gain_acc(const Ops& op, double* a, double2 Y, unsigned i) {
+  if constexpr (GK == SPG_SAT) {
+    energy_acc<N>(a, Y, i, op.tw);
+  } else {
+    const double ps = abs2(Y) * op.yb_pfac;
+    a[0] += op.yb_rs[i] * ps;
+    a[1] += op.yb_ip[i] * ps;
+  }
+}
+// The gain parameter of a linear call of step hl from the reduced sums: the
+// exponent g (SPG_SAT) or the inversion N2, which also advances the pump
+// power (SPG_YB, amplifier.py:382-388).
+template <int N, int GK, class D>
+__device__ __forceinline__ double gain_param(D& d, const double* a, double hl) {
+  const Ops& op = d.op;
+  if constexpr (GK == SPG_SAT) {
+    return sat_gain(op, energy_of<N>(op, a), hl);
+  } else {
+    const double pp = d.pp;
+    const double n2 = (((op.yb_rp * pp) / op.yb_psp + a[0]) / (1.0 + pp / op.yb_psp + a[1])) * op.yb_n;
+    d.pp = pp * exp((op.yb_ap * n2 - op.yb_ep) * (hl * 1e3));
+    return n2;
+  }
+}
+// Per-point operands of the gain factor and the factor itself (as
+// Dev::spectral forms it: the spectrum times E, then times the factor).
+template <int GK>
+__device__ __forceinline__ double2 gain_load(const Ops& op, unsigned i) {
+  if constexpr (GK == SPG_SAT) return make_double2(op.gprof[i], 0.0);
+  else return make_double2(op.yb_as[i], op.yb_es[i]);
+}
+template <int GK>
+__device__ __forceinline__ double gain_factor(double2 q, double gp, double hl) {
+  if constexpr (GK == SPG_SAT) {
+    return exp(gp * q.x);
+  } else {
+    const double gs = q.x * gp - q.y;
+    return exp(gs * (hl * 1e3) * 0.5);
+  }
+}
+
+// Entry of rk4ip_sp with a gain: A_I~ = factor E A~ into AIs. For SPG_SAT
+// the exponent g0 comes from A's time samples (the caller); for SPG_YB the
+// inversion comes from the spectrum A~ itself. x is the load buffer.
+template <int N, int T, int GK, int C, int KIND, bool YB>
+__device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], const double2* E,
+                                              const double2* As, double2* AIs, double g0, double hl, double* red) {
   constexpr int P = N / T;
+  const Ops& op = d.op;
+  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = As[T * k];
+  double gp = g0;
+  if constexpr (GK == SPG_YB) {
+    double a[SpSums<GK>::M] = {};
+#pragma unroll
+    for (int k = 0; k < P; ++k) gain_acc<N, GK>(op, a, x[k], t + T * k);
+    d.block_sum_n(a, red);
+    gp = gain_param<N, GK>(d, a, hl);
+  }
+#pragma unroll
+  for (int c = 0; c < P; c += C) {
+    double2 e[C], q[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      e[j] = E[T * (c + j)];
+      q[j] = gain_load<GK>(op, t + T * (c + j));
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) AIs[T * (c + j)] = cscale(cmul(e[j], x[c + j]), gain_factor<GK>(q[j], gp, hl));
+  }
+}
+
+// The stages of rk4ip_sp whose linear call carries a gain: the gain of L(y)
+// follows from y (its energy, or its spectral power), so the spectrum y is
+// formed and its sums reduced over the workgroup before it is multiplied by
+// the gain factor and E.
+//   ST = 1: y = h Nt(A) (x holds F+(x R(x)), or Nt(A) when !fresh);
+//           k0~ = factor E y into S, x <- conj(A_I~ + k0~/2).
+//   ST = 3: k2~ = h Nt; S <- A_I~ + (S + 2 k2~)/6; y = A_I~ + k2~;
+//           x <- conj(factor E y); gp4 <- the gain parameter of L(acc) (in
+//           the reference's call order: after L(A_I + k2)).
+//   ST = 4: x <- conj(k3~/6 + factor(gp4) E S), A'~ also to out.
+// red holds 2 SpSums<GK>::M NW doubles.
+template <int N, int T, int ST, int C, int GK, int KIND, bool YB>
+__device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], bool fresh, double2* nA,
+                                              double h, double hl, const double2* E, double2* S, const double2* AIs,
+                                              double2* out, double& gp4, double* red) {
+  constexpr int P = N / T;
+  constexpr int M = SpSums<GK>::M;
   constexpr double INVN = 1.0 / N;
   const Ops& op = d.op;
   const unsigned t = (unsigned)opaque((int)threadIdx.x);
-  const double* prof = d.mine(op.gprof);
   if constexpr (ST == 1) {
-    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double a[M] = {};
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       double2 n = x[k];
       if (fresh) n = cmul(op.nlmul[t + T * k], cscale(n, INVN));
       if (nA) nA[t + T * k] = n;
       x[k] = cscale(n, h);
-      energy_acc<N>(a, x[k], t + T * k, op.tw);
+      gain_acc<N, GK>(op, a, x[k], t + T * k);
     }
     d.block_sum_n(a, red);
-    const double g = sat_gain(op, energy_of<N>(op, a), hl);
+    const double gp = gain_param<N, GK>(d, a, hl);
 #pragma unroll
     for (int c = 0; c < P; c += C) {
-      double2 e[C], ai[C];
-      double pr[C];
+      double2 e[C], ai[C], q[C];
 #pragma unroll
       for (int j = 0; j < C; ++j) {
         e[j] = E[T * (c + j)];
         ai[j] = AIs[T * (c + j)];
-        pr[j] = prof[T * (c + j)];
+        q[j] = gain_load<GK>(op, t + T * (c + j));
       }
 #pragma unroll
       for (int j = 0; j < C; ++j) {
-        const double2 k0 = cscale(cmul(e[j], x[c + j]), exp(g * pr[j]));
+        const double2 k0 = cscale(cmul(e[j], x[c + j]), gain_factor<GK>(q[j], gp, hl));
         S[T * (c + j)] = k0;
         x[c + j] = conjg(cadd(ai[j], cscale(k0, 0.5)));
       }
     }
   } else if constexpr (ST == 3) {
-    double a[18];
-#pragma unroll
-    for (int m = 0; m < 18; ++m) a[m] = 0.0;
+    double a[2 * M] = {};
 #pragma unroll
     for (int c = 0; c < P; c += C) {
       double2 m[C], ai[C], b[C];
@@ -681,32 +757,42 @@ __device__ __forceinline__ void pw_sp_stage_sg(Dev<N, T, KIND, YB>& d, double2 (
         const double2 sn = cadd(ai[j], make_double2(q.x / 6.0, q.y / 6.0));
         S[T * (c + j)] = sn;
         x[c + j] = cadd(ai[j], v);
-        energy_acc<N>(a, x[c + j], t + T * (c + j), op.tw);
-        energy_acc<N>(a + 9, sn, t + T * (c + j), op.tw);
+        gain_acc<N, GK>(op, a, x[c + j], t + T * (c + j));
+        gain_acc<N, GK>(op, a + M, sn, t + T * (c + j));
       }
     }
     d.block_sum_n(a, red);
-    const double g = sat_gain(op, energy_of<N>(op, a), hl);
-    g4 = sat_gain(op, energy_of<N>(op, a + 9), hl);
+    const double gp = gain_param<N, GK>(d, a, hl);
+    gp4 = gain_param<N, GK>(d, a + M, hl);
 #pragma unroll
-    for (int k = 0; k < P; ++k) x[k] = conjg(cscale(cmul(E[T * k], x[k]), exp(g * prof[T * k])));
+    for (int c = 0; c < P; c += C) {
+      double2 e[C], q[C];
+#pragma unroll
+      for (int j = 0; j < C; ++j) {
+        e[j] = E[T * (c + j)];
+        q[j] = gain_load<GK>(op, t + T * (c + j));
+      }
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        x[c + j] = conjg(cscale(cmul(e[j], x[c + j]), gain_factor<GK>(q[j], gp, hl)));
+    }
   } else {
     static_assert(ST == 4, "stages 1, 3 and 4 carry a linear call");
 #pragma unroll
     for (int c = 0; c < P; c += C) {
-      double2 m[C], b[C], e[C];
-      double pr[C];
+      double2 m[C], b[C], e[C], q[C];
 #pragma unroll
       for (int j = 0; j < C; ++j) {
         m[j] = op.nlmul[t + T * (c + j)];
         b[j] = S[T * (c + j)];
         e[j] = E[T * (c + j)];
-        pr[j] = prof[T * (c + j)];
+        q[j] = gain_load<GK>(op, t + T * (c + j));
       }
 #pragma unroll
       for (int j = 0; j < C; ++j) {
         const double2 v = cscale(cmul(m[j], cscale(x[c + j], INVN)), h);
-        const double2 w = cadd(make_double2(v.x / 6.0, v.y / 6.0), cscale(cmul(e[j], b[j]), exp(g4 * pr[j])));
+        const double2 w =
+            cadd(make_double2(v.x / 6.0, v.y / 6.0), cscale(cmul(e[j], b[j]), gain_factor<GK>(q[j], gp4, hl)));
         if (out) out[T * (c + j)] = w;
         x[c + j] = conjg(w);
       }
@@ -809,15 +895,17 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 // in x and, when outS is given, spectral in outS (neither of them one of
 // the slots S_K0, S_K1, S_U used here). The kernel's LDS holds
 // sp_lds_doubles<N, T>() doubles.
-// SG: the saturable gain of amplifier.py:149-158 in every linear call (the
-// energies of the arrays the reference passes to them: A from its time
-// samples, the others from their spectra; Ops::simp_flat must hold; the
-// workgroup's red buffer holds 18 NW doubles).
-template <bool SG, int N, int T, int K, bool YB>
+// GK: the gain in every linear call (SpGain): SPG_SAT, the saturable gain
+// of amplifier.py:149-158 (energies of the arrays the reference passes: A
+// from its time samples, the others from their spectra; Ops::simp_flat must
+// hold); SPG_YB, the two-level Yb gain of amplifier.py:368-402 (from the
+// spectra of those arrays; the pump power advances in the reference's call
+// order). The workgroup's red buffer holds 18 NW doubles.
+template <int GK, int N, int T, int K, bool YB>
 __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
                                          double2* outS, const double2* E, double h, double2* ws, double2* nA,
                                          bool nA_ready, bool nA_store) {
-  static_assert(K == NL_ALL && !YB, "spectral-domain rk4ip: full nonlinear operator, no Yb gain");
+  static_assert(K == NL_ALL && YB == (GK == SPG_YB), "spectral-domain rk4ip: full nonlinear operator");
   using F = FFT<N, T>;
   constexpr int P = N / T;
   constexpr double INVN = Dev<N, T, K, YB>::INVN;
@@ -826,16 +914,20 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
   double2* sU = ws + S_U * N;     // the field x of the running N(x)
   const Ops& op = d.op;
   const double a1 = 1.0 - op.f_R;
+  constexpr int CW = P < 4 ? P : 4;
   int p0 = 0;
-  double g4 = 0.0;  // SG: gain exponent of L(acc), from stage 3
-  if constexpr (SG) {  // L(A): energy of A from its time samples (field.py:161-172)
-    d.load(x, in);
-    double e = 0.0;
+  double gp4 = 0.0;  // gain parameter of L(acc), from stage 3
+  if constexpr (GK != SPG_NONE) {
+    double g0 = 0.0;
+    if (GK == SPG_SAT || !nA_ready) d.load(x, in);
+    if constexpr (GK == SPG_SAT) {  // L(A): energy of A from its time samples (field.py:161-172)
+      double e = 0.0;
 #pragma unroll
-    for (int k = 0; k < P; ++k) e += op.simpw[d.idx(k)] * abs2(x[k]);
-    const double g0 = sat_gain(op, d.block_sum(e), 0.5 * h);
+      for (int k = 0; k < P; ++k) e += op.simpw[d.idx(k)] * abs2(x[k]);
+      g0 = sat_gain(op, d.block_sum(e), 0.5 * h);
+    }
     if (!nA_ready) raman_stage(d, x);
-    pw_sp_entry<P, T, true>(x, d.mine(E), d.mine(inS), d.mine(sAIs), g0, d.mine(op.gprof));
+    pw_sp_entry_g<N, T, GK, CW>(d, x, d.mine(E), d.mine(inS), d.mine(sAIs), g0, 0.5 * h, d.red);
     if (nA_ready) {
       d.load(x, nA);
       p0 = 1;
@@ -870,17 +962,17 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
       const bool fresh = !(st == 1 && nA_ready);
       double2* nAo = (st == 1 && nA_store) ? d.mine(nA) : nullptr;
       const double2* M = d.mine(op.nlmul);
-      constexpr int CW = P < 4 ? P : 4;
-      if constexpr (SG) {
+      if constexpr (GK != SPG_NONE) {
         const double hl = 0.5 * h;
         double2* out = outS ? d.mine(outS) : nullptr;
         if (st == 1)
-          pw_sp_stage_sg<N, T, 1, CW>(d, x, fresh, (st == 1 && nA_store) ? nA : nullptr, h, hl, d.mine(E), d.mine(sS),
-                                      d.mine(sAIs), nullptr, g4, d.red);
+          pw_sp_stage_g<N, T, 1, CW, GK>(d, x, fresh, nA_store ? nA : nullptr, h, hl, d.mine(E), d.mine(sS),
+                                         d.mine(sAIs), nullptr, gp4, d.red);
         else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
         else if (st == 3)
-          pw_sp_stage_sg<N, T, 3, CW>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr, g4, d.red);
-        else pw_sp_stage_sg<N, T, 4, CW>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), nullptr, out, g4, d.red);
+          pw_sp_stage_g<N, T, 3, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr, gp4,
+                                         d.red);
+        else pw_sp_stage_g<N, T, 4, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), nullptr, out, gp4, d.red);
         continue;
       }
       if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
