@@ -483,6 +483,9 @@ struct Launcher {
       hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.gain == GAIN_SATURABLE && o.simp_flat && o.nl_kind == NL_ALL)
       hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 2>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.gain == GAIN_YB && o.nl_kind == NL_ALL)
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+                         ws);
     else if (method == M_RK4IP)
       GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);

@@ -92,7 +92,8 @@ def adaptive_params(lib, fib, tol, traces=1, eta=5, total_steps=100):
     return p
 
 
-@pytest.mark.parametrize("N,length", [(1024, 1.0e-5), (8192, 1.0e-5)])
+@pytest.mark.parametrize("N,length", [(1024, 1.0e-5), (8192, 1.0e-5), (256, 1.0e-5), (512, 1.0e-5), (2048, 1.0e-5),
+                                      (4096, 1.0e-5), (16384, 2.0e-6)])
 def test_c2_adaptive_short(lib, N, length):
     """C2 physics (Raman + self-steepening, tol 1e-6) over the first part of
     the fibre: same attempt sequence and final field as the oracle."""
@@ -116,6 +117,28 @@ def test_c2_adaptive_short(lib, N, length):
     assert np.array_equal(out["log_accept"][0, :len(log)], ref_acc)
     assert dh < 1e-6
     assert err <= FIELD_TOL
+
+
+@pytest.mark.parametrize("N", [512, 8192])
+def test_c2_physics_fixed_steps(lib, N):
+    """Raman + self-steepening with fixed rk4ip steps (the spectral-domain
+    fixed-step launch): every trace and the output field against the
+    oracle's Fibre run."""
+    dom, A, fib = supercontinuum_c2(N)
+    steps, traces, length = 200, 10, 2.0e-6
+    f = O.Fibre(length=length, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True,
+                method="rk4ip", total_steps=steps, traces=traces)
+    ref = f(dom, A.copy())
+    ref_As = f.stepper.storage.As
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    got, tr, ntr = ctx.propagate_fixed(0, A, length / steps, steps, trace_every=steps // traces, max_traces=traces)
+    err = rel_l2(got, ref)
+    print(f"C2 physics fixed N={N}: rel L2 {err:.3e}, traces {ntr[0]}", flush=True)
+    assert err <= FIELD_TOL
+    assert ntr[0] == len(ref_As) == traces
+    for k in range(traces):
+        assert rel_l2(tr[0, k], ref_As[k]) <= FIELD_TOL
 
 
 def test_c2_replay_matches_oracle(lib):
