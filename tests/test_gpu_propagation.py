@@ -179,7 +179,8 @@ def amplifier_batch(lib, N, length, chirps):
 
 
 @pytest.mark.parametrize("N,length,chirps", [(2048, 5e-3, (-2.0, 0.5, 2.0)), (16384, 2e-4, (-2.0, 0.5, 2.0)),
-                                              (16384, 5e-3, (1.0,))], ids=["2048-full", "16384-short", "16384-full"])
+                                             (4096, 1e-3, (-2.0, 0.5)), (8192, 5e-4, (-1.0, 2.0)),
+                                              (16384, 5e-3, (1.0,))], ids=["2048-full", "16384-short", "4096-short", "8192-short", "16384-full"])
 def test_c4_amplifier_batch(lib, N, length, chirps):
     """C4 physics (Lorentzian saturable gain inside every linear sub-step,
     Raman, adaptive RK4IP): chirps in one launch, each instance with its own

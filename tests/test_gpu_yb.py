@@ -105,6 +105,21 @@ def test_fixed_steps_match_oracle(P, method, steps, traces):
     assert sp.fft_total == so.fft_total
 
 
+@pytest.mark.parametrize("samples", [4096, 8192])
+def test_fixed_rk4ip_grids(P, samples):
+    """Fixed-step rk4ip with the Yb gain on smaller grids (the spectral-domain
+    launch at 512 threads per instance): output field and pump power."""
+    od, pd_, A = domains(P, samples=samples)
+    h0 = CFG["length_km"] / CFG["total_steps"]
+    fo, fp = fibres(P, 60 * h0, "rk4ip", 60, 2)
+    out_o = fo(od, A.copy())
+    out_p = fp(pd_, A.copy())
+    e = rel_l2(out_p, out_o)
+    print(f"rk4ip {samples}: rel L2 {e:.2e}, Pp {fp.amplifier.Pp:.15g} (oracle {fo.amplifier.Pp:.15g})", flush=True)
+    assert e <= 1e-10
+    assert fp.amplifier.Pp == pytest.approx(fo.amplifier.Pp, rel=1e-10)
+
+
 def test_adaptive_matches_oracle(P):
     """ark4ip: step doubling calls the linear operator 12 times per attempt,
     in the reference's order (half step, fine step, coarse step), and every
