@@ -14,7 +14,8 @@ Printed: one JSON line with the metric, the roofline of the propagation
 kernel (algorithmic bytes = 72 FFTs x 32 B x N per attempted step, over the
 kernel's HIP-event duration, against 8 TB/s HBM) and a CPU baseline (the
 NumPy oracle on one host core, timed on a bounded sample of the same
-workload). Further objects report the other workloads of the engine:
+workload; beside it, "all_cores": the same sample in one single-threaded
+process per host core, up to 16). Further objects report the other workloads of the engine:
 "ensemble" (512 noise-seeded soliton realisations on 4096/8192/16384-sample
 grids in one graph-replayed launch with |g12| on the device), "amplifier"
 (64 chirped pulses through the 5 m saturable-gain amplifier, 16384 samples),
@@ -54,6 +55,8 @@ def parse():
     ap.add_argument("--length", type=float, default=1.5e-4, help="fibre length in km (default: the full 15 cm)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="processes of the all-core CPU baseline (at most the host's cores; 1: single core only)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the noise-ensemble measurement")
     ap.add_argument("--no-amplifier", action="store_true", help="skip the amplifier-batch measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
@@ -89,10 +92,9 @@ def cu_count():
     return 256
 
 
-def cpu_baseline(A0, length, attempts_full, budget_s):
-    """The oracle (NumPy restatement of pyofss) on one host core, on the first
-    attempted steps of the same propagation; scaled by the number of attempted
-    steps the full propagation takes."""
+def _oracle_attempts(A0, length, n):
+    """Time the oracle's first n attempted steps of the supercontinuum
+    propagation on one core: (seconds, attempts done)."""
     from threadpoolctl import threadpool_limits
 
     import oracle as O
@@ -100,23 +102,32 @@ def cpu_baseline(A0, length, attempts_full, budget_s):
 
     with threadpool_limits(1):
         dom = O.Domain(1, 8192, 12.5, C_NM_PS / 835.0)
+        f = O.Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
+                    method="ark4ip", local_error=1e-6)
+        f.linearity(dom)
+        f.nonlinearity(dom)
+        f.stepper.storage(dom)
+        f.stepper.storage.reset_fft_counter()
+        t = time.perf_counter()
+        f.stepper.adaptive_stepper(A0.copy(), None, max_attempts=n)
+        return time.perf_counter() - t, len(f.stepper.attempt_log)
 
-        def run(n):
-            f = O.Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
-                        method="ark4ip", local_error=1e-6)
-            f.linearity(dom)
-            f.nonlinearity(dom)
-            f.stepper.storage(dom)
-            f.stepper.storage.reset_fft_counter()
-            t = time.perf_counter()
-            f.stepper.adaptive_stepper(A0.copy(), None, max_attempts=n)
-            return time.perf_counter() - t, len(f.stepper.attempt_log)
 
-        t_probe, n_probe = run(8)
-        n = max(8, int(budget_s / (t_probe / max(n_probe, 1))))
-        elapsed, done = run(n)
+def _oracle_worker(args):
+    return _oracle_attempts(*args)
+
+
+def cpu_baseline(A0, length, attempts_full, budget_s, pool_workers):
+    """The oracle (NumPy restatement of pyofss) on one host core, on the first
+    attempted steps of the same propagation; scaled by the number of attempted
+    steps the full propagation takes. With pool_workers > 1 also the all-core
+    throughput: that many single-threaded worker processes run the same
+    sample side by side (each reports its own rate; the rates add)."""
+    t_probe, n_probe = _oracle_attempts(A0, length, 8)
+    n = max(8, int(budget_s / (t_probe / max(n_probe, 1))))
+    elapsed, done = _oracle_attempts(A0, length, n)
     per_attempt = elapsed / done
-    return {
+    out = {
         "value": 1.0 / (per_attempt * attempts_full),
         "unit": "propagations/s",
         "cores": 1,
@@ -125,6 +136,19 @@ def cpu_baseline(A0, length, attempts_full, budget_s):
                    f"({per_attempt * 1e3:.2f} ms/attempt), scaled to the {attempts_full:.0f} attempted steps of "
                    f"a full propagation"),
     }
+    if pool_workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(pool_workers) as pool:
+            res = pool.map(_oracle_worker, [(A0, length, n)] * pool_workers)
+        rate = sum(d / t for t, d in res)  # attempted steps per second over all workers
+        out["all_cores"] = {
+            "value": rate / attempts_full,
+            "unit": "propagations/s",
+            "cores": pool_workers,
+            "sample": (f"{pool_workers} single-threaded worker processes, each the same {n}-attempt sample "
+                       f"({1e3 * pool_workers / rate:.2f} ms/attempt per worker on average)"),
+        }
+    return out
 
 
 def ensemble_measurement():
@@ -381,7 +405,8 @@ def main():
     if not args.no_surrogate:
         line["surrogate"] = surrogate_measurement(args.surrogate_train, args.surrogate_steps, args.surrogate_length)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(inputs[0], args.length, float(np.mean(attempts)) / B, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(inputs[0], args.length, float(np.mean(attempts)) / B, args.cpu_seconds,
+                                            min(os.cpu_count() or 1, args.cpu_workers))
     print(json.dumps(line))
 
 
