@@ -875,8 +875,9 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 }
 
 // solver.py:339-354 rk4ip for the full Raman + self-steepening operator
-// (nonlinearity.py:201-207) without gain, carried in the spectral domain
-// between operators. The reference evaluates each linear call as
+// (nonlinearity.py:201-207), carried in the spectral domain between
+// operators (with a gain in the linear calls: GK below, where E stands for
+// the gain factor times E). The reference evaluates each linear call as
 // ifft(E fft(x)) and each N(x) as ifft(m fft(x R(x))), m = i gamma (1 +
 // omega ss); wherever a forward transform follows an inverse one the pair
 // cancels, so with Nt(x) = m fft(x R(x)) = fft(N(x)):
