@@ -469,6 +469,9 @@ struct Launcher {
       hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_NONE)
       hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_SATURABLE && o.simp_flat)
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1 + SPG_SAT>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+                         ws);
     else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();

@@ -203,6 +203,30 @@ def test_c4_amplifier_batch(lib, N, length, chirps):
         assert err <= FIELD_TOL
 
 
+@pytest.mark.parametrize("N", [2048, 16384])
+def test_c4_physics_fixed_steps(lib, N):
+    """The saturable-gain amplifier with fixed rk4ip steps (the spectral-domain
+    fixed-step launch with the gain): every trace and the output field."""
+    dom, A, fib = amplifier_c4(N, 0.5)
+    amp_args = fib["amplifier"]
+    steps, traces, length = 120, 4, 2e-4
+    f = O.Fibre(length=length, beta=fib["beta"], gamma=fib["gamma"], method="rk4ip", total_steps=steps,
+                traces=traces, use_all=True, small_signal_gain=amp_args["gain"], E_sat=amp_args["E_sat"],
+                lamb0=amp_args["lamb0"], bandwidth=amp_args["bandwidth"])
+    ref = f(dom, A.copy())
+    ref_As = f.stepper.storage.As
+    amp = oracle_amplifier(dom, length, **amp_args)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], use_all=True, amplifier=amp)
+    ctx = engine_context(dom, lin, nl, 1, amplifier=amp)
+    got, tr, ntr = ctx.propagate_fixed(0, A, length / steps, steps, trace_every=steps // traces, max_traces=traces)
+    err = rel_l2(got, ref)
+    print(f"C4 physics fixed N={N}: rel L2 {err:.3e}, traces {ntr[0]}", flush=True)
+    assert err <= FIELD_TOL
+    assert ntr[0] == len(ref_As) == traces
+    for k in range(traces):
+        assert rel_l2(tr[0, k], ref_As[k]) <= FIELD_TOL
+
+
 def test_adaptive_nan_input_raises_small_step(lib):
     """stepper.py:234-237, 349-351: NaN fields are rejected until h < 1e-37."""
     dom, A, fib = soliton_c1(1024)
