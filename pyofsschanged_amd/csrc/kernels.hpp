@@ -301,6 +301,10 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         sd += abs2(df);
         sf += abs2(f);
         if (!(isfinite(f.x) && isfinite(f.y))) bad = 1.0;
+        // SP >= 2: the local extrapolation of stepper.py:336-338 formed here
+        // (a rejected attempt drops x, the next one starts from sA); this
+        // helps the gain kernels and costs the gain-free one registers
+        if constexpr (SP >= 2) x[k] = csub(cscale(f, prm.f_alpha), cscale(x[k], prm.f_beta));
       }
       bad = d.block_sum(bad);
       sd = d.block_sum(sd);
@@ -329,9 +333,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         z += h_temp;
         ++n_acc;
         // local extrapolation, stepper.py:336-338
+        if constexpr (SP < 2) {
 #pragma unroll
-        for (int k = 0; k < P; ++k)
-          x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
+          for (int k = 0; k < P; ++k)
+            x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
+        }
         d.store(sA, x);
         nA_valid = false;
         if (z > z_ignore) {  // stepper.py:342-345
