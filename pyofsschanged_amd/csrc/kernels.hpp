@@ -301,10 +301,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         sd += abs2(df);
         sf += abs2(f);
         if (!(isfinite(f.x) && isfinite(f.y))) bad = 1.0;
-        // SP >= 2: the local extrapolation of stepper.py:336-338 formed here
-        // (a rejected attempt drops x, the next one starts from sA); this
-        // helps the gain kernels and costs the gain-free one registers
-        if constexpr (SP >= 2) x[k] = csub(cscale(f, prm.f_alpha), cscale(x[k], prm.f_beta));
+        // Gain and Kerr kernels: the local extrapolation of stepper.py:336-338
+        // formed here (a rejected attempt drops x, the next one starts from
+        // sA); it saves a pass over sH there and costs the gain-free Raman
+        // kernels registers
+        if constexpr (SP >= 2 || K == NL_KERR) x[k] = csub(cscale(f, prm.f_alpha), cscale(x[k], prm.f_beta));
       }
       bad = d.block_sum(bad);
       sd = d.block_sum(sd);
@@ -333,7 +334,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         z += h_temp;
         ++n_acc;
         // local extrapolation, stepper.py:336-338
-        if constexpr (SP < 2) {
+        if constexpr (!(SP >= 2 || K == NL_KERR)) {
 #pragma unroll
           for (int k = 0; k < P; ++k)
             x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
