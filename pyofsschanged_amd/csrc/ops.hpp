@@ -95,12 +95,6 @@ __device__ __forceinline__ double2 sel(bool c, double2 a, double2 b) {
   return make_double2(c ? a.x : b.x, c ? a.y : b.y);
 }
 
-// Pointwise passes of rk4ip_sp that load and store different arrays. Each
-// pointer is offset to the calling thread's first owned point and the points
-// follow at strides of T. The compiler keeps loads below earlier stores
-// unless it can prove they differ, which it does not here, so a pass written
-// point by point waits for one memory round trip per point; these passes
-// load a block of C points of every input first, then compute and store it.
 // Energy of a field (field.py:161-172, the Simpson sum of |A|^2 over t) from
 // its spectrum X = fft(A) when the weights are simp_c except at points 0,
 // 1, n-2, n-1 (Ops::simp_flat): E = simp_c n sum |X_k|^2 + sum_j simp_d[j]
@@ -136,6 +130,12 @@ __device__ __forceinline__ double sat_gain(const Ops& op, double En, double hl) 
   return G0 / (1.0 + En / op.esat);
 }
 
+// Pointwise passes of rk4ip_sp that load and store different arrays. Each
+// pointer is offset to the calling thread's first owned point and the points
+// follow at strides of T. The compiler keeps loads below earlier stores
+// unless it can prove they differ, which it does not here, so a pass written
+// point by point waits for one memory round trip per point; these passes
+// load a block of C points of every input first, then compute and store it.
 // Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer).
 template <int P, int T>
 __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
