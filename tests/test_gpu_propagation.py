@@ -238,6 +238,33 @@ def test_adaptive_nan_input_raises_small_step(lib):
     assert out["results"][0]["status"] == lib.STATUS_SMALL_STEP
 
 
+def test_spectral_adaptive_nan_and_large_batch(lib):
+    """The spectral-domain adaptive kernel (Raman + self-steepening): a NaN
+    field ends in SMALL_STEP (stepper.py:234-237, 349-351) beside regular
+    instances, and a batch larger than the CU count (several workgroups per
+    CU over the launch) gives every instance exactly its single-run result."""
+    dom, A, fib = supercontinuum_c2(1024)
+    fib["length"] = 2e-6
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    rng = np.random.default_rng(3)
+    B = 300
+    batch = np.stack([A * (1 + 0.002 * (k % 7)) + 1e-2 * rng.standard_normal(A.size) for k in range(B)])
+    batch[17, 5] = np.nan
+    params = adaptive_params(lib, fib, 1e-6)
+    out = ctx.propagate_adaptive(0, batch, params)
+    st = [r["status"] for r in out["results"]]
+    assert st[17] == lib.STATUS_SMALL_STEP
+    assert all(s == 0 for k, s in enumerate(st) if k != 17)
+    for k in (0, 1, 150, 299):
+        one = ctx.propagate_adaptive(0, batch[k], params)
+        assert one["results"][0]["n_attempts"] == out["results"][k]["n_attempts"]
+        assert np.array_equal(one["A"][0] if one["A"].ndim == 2 else one["A"], out["A"][k])
+    ref, f = adaptive_oracle(dom, batch[150], fib, "ark4ip", 1e-6)
+    assert out["results"][150]["n_attempts"] == len(f.stepper.attempt_log)
+    assert rel_l2(out["A"][150], ref) <= FIELD_TOL
+
+
 def test_adaptive_steps_max(lib):
     dom, A, fib = soliton_c1(1024)
     lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
