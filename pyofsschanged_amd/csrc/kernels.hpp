@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
   d.pump_begin(b);
   long long n_tr = 0;
   for (long long i = 0; i < prm.nsteps; ++i) {
-    if constexpr (SP != 0) rk4ip_sp<SP - 1>(d, x, sA, sAs, sAs, ops.efix, prm.h, ws, nullptr, false, false);
+    if constexpr (SP != 0) rk4ip_sp<SP - 1>(d, x, sA, sAs, sAs, ops.efix, false, prm.h, ws, nullptr, false, false);
     else integrator<METHOD>(d, x, sA, ops.efix, prm.h, ws);
     d.store(sA, x);
     if (prm.trace_every > 0 && (i % prm.trace_every) == 0) {  // stepper.py:206-209
@@ -269,7 +269,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       }
       const double h_half = 0.5 * h;
       if constexpr (SP != 0) {
-        d.make_exp_pair(sEQ, sEH, 0.5 * h_half);
+        d.make_exp(sEQ, 0.5 * h_half);  // the coarse step squares it where it is read
       } else {
         d.make_exp(sEH, 0.5 * h);
         d.make_exp(sEQ, 0.5 * h_half);
@@ -285,8 +285,8 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         // attempt from the same accepted field
         const bool from_a = (c != 1);
         if constexpr (SP != 0)
-          rk4ip_sp<SP - 1>(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, E, c == 2 ? h : h_half, ws, sNA,
-                           from_a && nA_valid, from_a && !nA_valid);
+          rk4ip_sp<SP - 1>(d, x, src, c == 1 ? sHs : sAs, c == 0 ? sHs : nullptr, sEQ, c == 2, c == 2 ? h : h_half, ws,
+                           sNA, from_a && nA_valid, from_a && !nA_valid);
         else
           integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
         if (from_a) nA_valid = true;

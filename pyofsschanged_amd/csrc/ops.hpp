@@ -88,6 +88,11 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
 
 __device__ __forceinline__ double abs2(double2 a) { return a.x * a.x + a.y * a.y; }
 
+// The dispersion factor of a linear call: E as stored, or its square (the
+// coarse step's exp(factor h/2) from the stored exp(factor h/4): the same
+// product the per-attempt set-up would store).
+__device__ __forceinline__ double2 efac(double2 e, bool esq) { return esq ? cmul(e, e) : e; }
+
 // Component-wise select. A ternary between two double2 array elements can be
 // lowered to a select of their addresses, which pins both arrays in scratch
 // memory; selecting the scalars keeps them in registers.
@@ -138,11 +143,12 @@ __device__ __forceinline__ double sat_gain(const Ops& op, double En, double hl) 
 // load a block of C points of every input first, then compute and store it.
 // Entry of rk4ip_sp: A_I~ = E A~ into AIs (x is the load buffer).
 template <int P, int T>
-__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, const double2* As, double2* AIs) {
+__device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, bool esq, const double2* As,
+                                            double2* AIs) {
   double2 e[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    e[k] = E[T * k];
+    e[k] = efac(E[T * k], esq);
     x[k] = As[T * k];
   }
 #pragma unroll
@@ -155,8 +161,8 @@ __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, c
 // Blocks of C points.
 template <int P, int T, int ST, int C>
 __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2* nA, double h, double invn,
-                                            const double2* M, const double2* E, double2* S, const double2* AIs,
-                                            double2* out) {
+                                            const double2* M, const double2* E, bool esq, double2* S,
+                                            const double2* AIs, double2* out) {
   static_assert(P % C == 0, "block size");
 #pragma unroll
   for (int c = 0; c < P; c += C) {
@@ -164,7 +170,7 @@ __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2
 #pragma unroll
     for (int j = 0; j < C; ++j) {
       if (fresh) m[j] = M[T * (c + j)];
-      if (ST != 2) e[j] = E[T * (c + j)];
+      if (ST != 2) e[j] = efac(E[T * (c + j)], esq);
       if (ST != 1) b[j] = S[T * (c + j)];
       if (ST != 4) a[j] = AIs[T * (c + j)];
     }
@@ -524,17 +530,6 @@ struct Dev {
 
   // E <- exp(lin * h) at owned indices (linearity.py:242, 246)
   __device__ __forceinline__ void make_exp(double2* __restrict__ E, double h) const { make_exp_of(E, op.lin, h); }
-  // E1 <- exp(lin * h) and E2 <- E1^2 = exp(lin * 2h) (to rounding: one
-  // complex exponential per point for the two step sizes of an attempt)
-  __device__ __forceinline__ void make_exp_pair(double2* __restrict__ E1, double2* __restrict__ E2, double h) const {
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-      const double2 f = op.lin[idx(k)];
-      const double2 e = cexp(make_double2(f.x * h, f.y * h));
-      E1[idx(k)] = e;
-      E2[idx(k)] = cmul(e, e);
-    }
-  }
   // exp(factor * h) of a given linear factor (the WDM channels have two)
   __device__ __forceinline__ void make_exp_of(double2* __restrict__ E, const double2* __restrict__ fac, double h) const {
 #pragma unroll
@@ -663,7 +658,7 @@ __device__ __forceinline__ double gain_factor(double2 q, double gp, double hl) {
 // the exponent g0 comes from A's time samples (the caller); for SPG_YB the
 // inversion comes from the spectrum A~ itself. x is the load buffer.
 template <int N, int T, int GK, int C, int KIND, bool YB>
-__device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], const double2* E,
+__device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], const double2* E, bool esq,
                                               const double2* As, double2* AIs, double g0, double hl, double* red) {
   constexpr int P = N / T;
   const Ops& op = d.op;
@@ -683,7 +678,7 @@ __device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&
     double2 e[C], q[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) {
-      e[j] = E[T * (c + j)];
+      e[j] = efac(E[T * (c + j)], esq);
       q[j] = gain_load<GK>(op, t + T * (c + j));
     }
 #pragma unroll
@@ -704,8 +699,8 @@ __device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&
 // red holds 2 SpSums<GK>::M NW doubles.
 template <int N, int T, int ST, int C, int GK, int KIND, bool YB>
 __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], bool fresh, double2* nA,
-                                              double h, double hl, const double2* E, double2* S, const double2* AIs,
-                                              double2* out, double& gp4, double* red) {
+                                              double h, double hl, const double2* E, bool esq, double2* S,
+                                              const double2* AIs, double2* out, double& gp4, double* red) {
   constexpr int P = N / T;
   constexpr int M = SpSums<GK>::M;
   constexpr double INVN = 1.0 / N;
@@ -728,7 +723,7 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&
       double2 e[C], ai[C], q[C];
 #pragma unroll
       for (int j = 0; j < C; ++j) {
-        e[j] = E[T * (c + j)];
+        e[j] = efac(E[T * (c + j)], esq);
         ai[j] = AIs[T * (c + j)];
         q[j] = gain_load<GK>(op, t + T * (c + j));
       }
@@ -769,7 +764,7 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&
       double2 e[C], q[C];
 #pragma unroll
       for (int j = 0; j < C; ++j) {
-        e[j] = E[T * (c + j)];
+        e[j] = efac(E[T * (c + j)], esq);
         q[j] = gain_load<GK>(op, t + T * (c + j));
       }
 #pragma unroll
@@ -785,7 +780,7 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&
       for (int j = 0; j < C; ++j) {
         m[j] = op.nlmul[t + T * (c + j)];
         b[j] = S[T * (c + j)];
-        e[j] = E[T * (c + j)];
+        e[j] = efac(E[T * (c + j)], esq);
         q[j] = gain_load<GK>(op, t + T * (c + j));
       }
 #pragma unroll
@@ -904,7 +899,7 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 // order). The workgroup's red buffer holds 18 NW doubles.
 template <int GK, int N, int T, int K, bool YB>
 __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
-                                         double2* outS, const double2* E, double h, double2* ws, double2* nA,
+                                         double2* outS, const double2* E, bool esq, double h, double2* ws, double2* nA,
                                          bool nA_ready, bool nA_store) {
   static_assert(K == NL_ALL && YB == (GK == SPG_YB), "spectral-domain rk4ip: full nonlinear operator");
   using F = FFT<N, T>;
@@ -928,19 +923,19 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
       g0 = sat_gain(op, d.block_sum(e), 0.5 * h);
     }
     if (!nA_ready) raman_stage(d, x);
-    pw_sp_entry_g<N, T, GK, CW>(d, x, d.mine(E), d.mine(inS), d.mine(sAIs), g0, 0.5 * h, d.red);
+    pw_sp_entry_g<N, T, GK, CW>(d, x, d.mine(E), esq, d.mine(inS), d.mine(sAIs), g0, 0.5 * h, d.red);
     if (nA_ready) {
       d.load(x, nA);
       p0 = 1;
     }
   } else if (nA_ready) {  // stage 1 starts from the stored Nt(A)
-    pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
+    pw_sp_entry<P, T>(x, d.mine(E), esq, d.mine(inS), d.mine(sAIs));
     d.load(x, nA);
     p0 = 1;
   } else {  // the product after the convolution reads A from `in`
     d.load(x, in);
     raman_stage(d, x);
-    pw_sp_entry<P, T>(x, d.mine(E), d.mine(inS), d.mine(sAIs));
+    pw_sp_entry<P, T>(x, d.mine(E), esq, d.mine(inS), d.mine(sAIs));
   }
   bool skip = nA_ready;  // the next pass has no transform (Nt(A) was loaded)
 #pragma unroll 1
@@ -967,19 +962,19 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
         const double hl = 0.5 * h;
         double2* out = outS ? d.mine(outS) : nullptr;
         if (st == 1)
-          pw_sp_stage_g<N, T, 1, CW, GK>(d, x, fresh, nA_store ? nA : nullptr, h, hl, d.mine(E), d.mine(sS),
+          pw_sp_stage_g<N, T, 1, CW, GK>(d, x, fresh, nA_store ? nA : nullptr, h, hl, d.mine(E), esq, d.mine(sS),
                                          d.mine(sAIs), nullptr, gp4, d.red);
-        else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
+        else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, false, d.mine(sS), d.mine(sAIs), nullptr);
         else if (st == 3)
-          pw_sp_stage_g<N, T, 3, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr, gp4,
+          pw_sp_stage_g<N, T, 3, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), esq, d.mine(sS), d.mine(sAIs), nullptr, gp4,
                                          d.red);
-        else pw_sp_stage_g<N, T, 4, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), d.mine(sS), nullptr, out, gp4, d.red);
+        else pw_sp_stage_g<N, T, 4, CW, GK>(d, x, true, nullptr, h, hl, d.mine(E), esq, d.mine(sS), nullptr, out, gp4, d.red);
         continue;
       }
-      if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
-      else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, d.mine(sS), d.mine(sAIs), nullptr);
-      else if (st == 3) pw_sp_stage<P, T, 3, CW>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), d.mine(sAIs), nullptr);
-      else pw_sp_stage<P, T, 4, CW>(x, true, nullptr, h, INVN, M, d.mine(E), d.mine(sS), nullptr, outS ? d.mine(outS) : nullptr);
+      if (st == 1) pw_sp_stage<P, T, 1, CW>(x, fresh, nAo, h, INVN, M, d.mine(E), esq, d.mine(sS), d.mine(sAIs), nullptr);
+      else if (st == 2) pw_sp_stage<P, T, 2, CW>(x, true, nullptr, h, INVN, M, nullptr, false, d.mine(sS), d.mine(sAIs), nullptr);
+      else if (st == 3) pw_sp_stage<P, T, 3, CW>(x, true, nullptr, h, INVN, M, d.mine(E), esq, d.mine(sS), d.mine(sAIs), nullptr);
+      else pw_sp_stage<P, T, 4, CW>(x, true, nullptr, h, INVN, M, d.mine(E), esq, d.mine(sS), nullptr, outS ? d.mine(outS) : nullptr);
     } else {  // inverse transform done: the next stage's field (or A')
 #pragma unroll
       for (int k = 0; k < P; ++k) x[k] = conjg(x[k]);
