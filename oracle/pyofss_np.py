@@ -40,9 +40,15 @@ Pins (tests/test_oracle_*.py):
   Raman convolution           -> brute-force O(N^2) causal convolution, unit area
   self-steepening factor      -> pure tone eigenvalue
   full nonlinearity           -> reduces to Kerr / Kerr+SS at f_R=0; photon number
-  adaptive stepper            -> controller arithmetic replayed by hand on a tiny
-                                 case; convergence to the closed-form soliton
-  amplifier                   -> small-signal gain closed form
+  adaptive stepper            -> attempt-by-attempt fixtures written by the
+                                 reference's own stepper.py (scalar ODE bit for
+                                 bit; Kerr, Raman + self-steepening and gain
+                                 fibres), planted controller mistakes failing
+                                 them, fifth-order convergence of the local
+                                 extrapolation (tests/test_oracle_adaptive.py)
+  amplifier                   -> small-signal gain closed form; flat saturated
+                                 gain converging to the Lambert-W solution of
+                                 dE/dz = g0 E / (1 + E/E_sat)
   two-level Yb gain           -> the df_results values printed by
                                  examples/notebooks/nonlinear_attractor.ipynb
                                  (first five traces, 1e-11; tests/test_yb_oracle.py),
