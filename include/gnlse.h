@@ -212,6 +212,25 @@ int gnlse_memcpy_dtod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes)
 int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the propagation of gnlse_propagate_adaptive_device with
+ * the per-instance workspace in a library-owned device array of
+ * GNLSE_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead
+ * of context buffers, and the per-instance results written to device memory
+ * d_results (batch entries). The call allocates nothing, enqueues one launch
+ * on the context's stream and returns without waiting (gnlse_memcpy_dtoh on
+ * the same context waits for it). Rejected with GNLSE_ERR_ARG when batch x
+ * 176 n bytes (twice that for WDM fields) exceeds the array: 45 instances at
+ * n = 8192. Every call on one GPU uses that GPU's one array: calls that
+ * overlap on a GPU (different contexts or threads) race. The operator
+ * set-up calls (gnlse_set_grid, _linear, _nonlinear, _gain, _pump) still
+ * allocate and must come first. */
+#define GNLSE_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int gnlse_propagate_adaptive_device_noalloc(gnlse_ctx* ctx, int method, int64_t batch,
+                                            const gnlse_adaptive_params* p, const double* d_A_in, double* d_A_out,
+                                            gnlse_instance_result* d_results);
+
 /* Trace characteristics, Storage.update_characts (pyofss/modules/storage.py:
  * 194-209) for `count` fields A (host, complex128, time domain) in one
  * launch. out[4k..4k+3] = energy (sum of simpson_w * |A|^2, simpson_w the

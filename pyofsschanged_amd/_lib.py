@@ -108,6 +108,7 @@ SIGNATURES = {
     "gnlse_memcpy_dtoh": [_P, _P, _P, _I64],
     "gnlse_memcpy_dtod": [_P, _P, _P, _I64],
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
+    "gnlse_propagate_adaptive_device_noalloc": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
     "gnlse_trace_powers": [_P, _I64, _P, _I64, _INT, _P, _P],
@@ -427,6 +428,12 @@ class Context:
         self._pump_out(batch)
         return [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
                  "n_traces": r.n_traces, "status": r.status} for r in res]
+
+    def propagate_adaptive_device_noalloc(self, method, batch, params, d_in, d_out, d_res):
+        """Enqueue the no-allocation propagation; d_res receives batch
+        InstanceResult records in device memory (read them with dtoh)."""
+        self._call("gnlse_propagate_adaptive_device_noalloc", int(method), int(batch), ctypes.byref(params), d_in,
+                   d_out, d_res)
 
     def last_kernel_ms(self):
         v = ctypes.c_double()

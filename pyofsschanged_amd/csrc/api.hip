@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,30 @@ const LaunchTable* find_table(long long n) {
 using namespace gn;
 
 thread_local std::string g_err;
+
+// Workspace of gnlse_propagate_adaptive_device_noalloc: a library-owned
+// device array, one copy per GPU, held until the process ends. In host code
+// the name `scratch` is not a device address; gnlse_ctx_create looks up the
+// address of the current device's copy once (noalloc_scratch) and every
+// launch takes that pointer.
+static __device__ __align__(256) unsigned char scratch[GNLSE_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device
+
+// Address of this device's `scratch` (the device must be current).
+static int noalloc_scratch(int device, void** ptr) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GNLSE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return GNLSE_OK;
+}
 
 static int timed_begin(gnlse_ctx* c) {
   HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -138,8 +163,11 @@ int gnlse_ctx_create(int device, gnlse_ctx** out) {
     return fail(GNLSE_ERR_HIP, std::string("no HIP device available: ") + hipGetErrorString(e));
   if (device < 0 || device >= n) return fail(GNLSE_ERR_ARG, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  void* sp = nullptr;
+  RET_IF(noalloc_scratch(device, &sp));
   gnlse_ctx* c = new gnlse_ctx();
   c->device = device;
+  c->scratch = sp;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -611,6 +639,30 @@ int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, con
   HIPCHK(hipMemcpyAsync(results, c->res.p, (size_t)batch * sizeof(gnlse_instance_result), hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+int gnlse_propagate_adaptive_device_noalloc(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
+                                            const double* d_A_in, double* d_A_out, gnlse_instance_result* d_results) {
+  RET_IF(check_ready(c, true, true));
+  AdaptiveParams prm;
+  RET_IF(adaptive_params(method, p, prm));
+  if (batch <= 0 || batch > 0x7fffffff || !d_A_in || !d_A_out || !d_results)
+    return fail(GNLSE_ERR_ARG, "invalid arguments");
+  RET_IF(check_pump(c, batch));
+  const size_t per = (size_t)fields_per_instance(c) * NSLOT * c->n * sizeof(double2);
+  if ((size_t)batch * per > (size_t)GNLSE_NOALLOC_SCRATCH_BYTES)
+    return fail(GNLSE_ERR_ARG, "batch " + std::to_string(batch) + " needs " + std::to_string((size_t)batch * per) +
+                                   " bytes of workspace; the no-allocation entry point holds " +
+                                   std::to_string((size_t)GNLSE_NOALLOC_SCRATCH_BYTES) + " (at most " +
+                                   std::to_string((size_t)GNLSE_NOALLOC_SCRATCH_BYTES / per) +
+                                   " instances on this grid)");
+  if (!c->scratch) return fail(GNLSE_ERR_STATE, "no workspace address for this device");
+  HIPCHK(hipSetDevice(c->device));
+  prm.res = d_results;
+  const hipError_t e = c->tab->adaptive(c->stream, c->ops, method, prm, batch, (const double2*)d_A_in,
+                                        (double2*)d_A_out, (double2*)c->scratch);
+  if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return GNLSE_OK;
 }
 
