@@ -11,11 +11,15 @@ local error 1e-6) through the whole fibre. Inputs are resident in HBM when
 the timed region starts; every instance runs the full adaptive propagation.
 
 Printed: one JSON line with the metric, the roofline of the propagation
-kernel (algorithmic bytes = 72 FFTs x 32 B x N per attempted step, over the
-kernel's HIP-event duration, against 8 TB/s HBM) and a CPU baseline (the
-NumPy oracle on one host core, timed on a bounded sample of the same
-workload; beside it, "all_cores": the same sample in one single-threaded
-process per host core, up to 16). Further objects report the other workloads of the engine:
+kernel (the reference's fft_counter work, 72 FFTs x 5 N log2 N flops per
+attempted step, over the kernel's HIP-event duration, against the 78.6
+TFLOP/s FP64 peak; beside it the measured HBM traffic of the same launch
+against 8 TB/s and the fft_counter work as bytes), a CPU baseline (the NumPy
+oracle on one host core running the whole propagation of realisation 0,
+unscaled; beside it, "all_cores": a bounded sample in one single-threaded
+process per core of the GPU's 16-core share, scaled) and "error", the
+relative L2 field error of realisation 0 against that oracle run. Further
+objects report the other workloads of the engine:
 "ensemble" (512 noise-seeded soliton realisations on 4096/8192/16384-sample
 grids in one graph-replayed launch with |g12| on the device), "amplifier"
 (64 chirped pulses through the 5 m saturable-gain amplifier, 16384 samples),
@@ -33,6 +37,7 @@ gated and audited fields).
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -42,6 +47,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "GNLSE propagations/s at 2^13 grid (FP64) on 1 GPU; surrogate speedup at 1% err"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64
 FFTS_PER_ATTEMPT = 72  # 3 RK4IP calls x (4 linear x 2 + 4 Raman/SS nonlinear x 4) FFTs
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "supercontinuum_8192_traffic.json")
 
@@ -92,9 +98,10 @@ def cu_count():
     return 256
 
 
-def _oracle_attempts(A0, length, n):
+def _oracle_attempts(A0, length, n, want_field=False):
     """Time the oracle's first n attempted steps of the supercontinuum
-    propagation on one core: (seconds, attempts done)."""
+    propagation on one core (n = None: the whole propagation): (seconds,
+    attempts done) and, with want_field, the field reached."""
     from threadpoolctl import threadpool_limits
 
     import oracle as O
@@ -109,34 +116,59 @@ def _oracle_attempts(A0, length, n):
         f.stepper.storage(dom)
         f.stepper.storage.reset_fft_counter()
         t = time.perf_counter()
-        f.stepper.adaptive_stepper(A0.copy(), None, max_attempts=n)
-        return time.perf_counter() - t, len(f.stepper.attempt_log)
+        A = f.stepper.adaptive_stepper(A0.copy(), None, max_attempts=n)
+        dt = time.perf_counter() - t
+        if want_field:
+            return dt, len(f.stepper.attempt_log), A
+        return dt, len(f.stepper.attempt_log)
 
 
 def _oracle_worker(args):
     return _oracle_attempts(*args)
 
 
-def cpu_baseline(A0, length, attempts_full, budget_s, pool_workers):
-    """The oracle (NumPy restatement of pyofss) on one host core, on the first
-    attempted steps of the same propagation; scaled by the number of attempted
-    steps the full propagation takes. With pool_workers > 1 also the all-core
-    throughput: that many single-threaded worker processes run the same
-    sample side by side (each reports its own rate; the rates add)."""
-    t_probe, n_probe = _oracle_attempts(A0, length, 8)
-    n = max(8, int(budget_s / (t_probe / max(n_probe, 1))))
-    elapsed, done = _oracle_attempts(A0, length, n)
-    per_attempt = elapsed / done
+def host_description():
+    """CPU model (lscpu "Model name") and the host's logical CPU count."""
+    model = None
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in txt.splitlines():
+            if ln.strip().startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(A0, length, gpu_out0, gpu_attempts0, attempts_full, budget_s, pool_workers):
+    """The oracle (NumPy restatement of pyofss) on one host core: the whole
+    propagation of realisation 0, unscaled, whose output also gives that
+    realisation's error (GPU vs oracle, relative L2 over the full fibre).
+    With pool_workers > 1 also the all-core throughput: that many
+    single-threaded worker processes each run the first attempted steps of
+    the same propagation (a bounded sample of about budget_s seconds) side
+    by side; their attempt rates add and are scaled by the attempts of a full
+    propagation. Returns (cpu_baseline, error)."""
+    elapsed, done, ref = _oracle_attempts(A0, length, None, want_field=True)
+    err = float(np.linalg.norm(gpu_out0 - ref) / np.linalg.norm(ref))
+    error = {"realisation": 0, "rel_l2_vs_oracle": err, "attempts": int(gpu_attempts0), "oracle_attempts": done,
+             "bound": 1e-9,
+             "note": "relative L2 field error of the GPU's realisation 0 against the oracle over the full 15 cm; "
+                     "every realisation of the launch runs the same kernel"}
     out = {
-        "value": 1.0 / (per_attempt * attempts_full),
+        "value": 1.0 / elapsed,
         "unit": "propagations/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"NumPy oracle, first {done} attempted steps of one realisation in {elapsed:.1f} s "
-                   f"({per_attempt * 1e3:.2f} ms/attempt), scaled to the {attempts_full:.0f} attempted steps of "
-                   f"a full propagation"),
+        "sample": (f"NumPy oracle, the whole propagation of realisation 0 ({done} attempted steps) in "
+                   f"{elapsed:.1f} s on one core, unscaled"),
+        "host": host_description(),
     }
     if pool_workers > 1:
+        t_probe, n_probe = _oracle_attempts(A0, length, 8)
+        n = max(8, int(budget_s / (t_probe / max(n_probe, 1))))
         import multiprocessing as mp
         with mp.get_context("spawn").Pool(pool_workers) as pool:
             res = pool.map(_oracle_worker, [(A0, length, n)] * pool_workers)
@@ -145,10 +177,12 @@ def cpu_baseline(A0, length, attempts_full, budget_s, pool_workers):
             "value": rate / attempts_full,
             "unit": "propagations/s",
             "cores": pool_workers,
-            "sample": (f"{pool_workers} single-threaded worker processes, each the same {n}-attempt sample "
-                       f"({1e3 * pool_workers / rate:.2f} ms/attempt per worker on average)"),
+            "sample": (f"{pool_workers} single-threaded worker processes (the CPU share of one GPU on this host), "
+                       f"each the first {n} attempted steps of realisation 0 "
+                       f"({1e3 * pool_workers / rate:.2f} ms/attempt per worker on average), scaled to the "
+                       f"{attempts_full:.0f} attempted steps of a full propagation"),
         }
-    return out
+    return out, error
 
 
 def ensemble_measurement():
@@ -349,16 +383,21 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = B * args.steps / wall
     mean_kernel_s = 1e-3 * float(np.mean(kernel_ms))
-    bytes_per_launch = float(np.mean(attempts)) * FFTS_PER_ATTEMPT * 32 * N
-    achieved = bytes_per_launch / mean_kernel_s / 1e9
-    # measured HBM bytes per attempted step (rocprofv3 FETCH_SIZE / WRITE_SIZE,
-    # copy-calibrated, same kernel and batch) times this launch's attempts
+    attempts_launch = float(np.mean(attempts))
+    # the reference's accounting (field.fft_counter): 72 N-point FFTs per
+    # attempted step, each 5 N log2 N flops and one HBM read + write of the field
+    flops_per_launch = attempts_launch * FFTS_PER_ATTEMPT * 5.0 * N * math.log2(N)
+    fft_bytes_per_launch = attempts_launch * FFTS_PER_ATTEMPT * 32 * N
+    achieved_tflops = flops_per_launch / mean_kernel_s / 1e12
+    # measured HBM bytes of the same kernel and batch over the full fibre
+    # (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), per attempted step,
+    # times this launch's attempts (the inputs are seeded: the same attempts)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as fh:
             t = json.load(fh)
-        if t.get("batch", B) == B and "hbm_bytes_per_attempt" in t:
-            traffic = t["hbm_bytes_per_attempt"] * float(np.mean(attempts))
+        if t.get("batch", B) == B and t.get("length_km") == args.length and "hbm_bytes_per_attempt" in t:
+            traffic = t["hbm_bytes_per_attempt"] * attempts_launch
 
     line = {
         "metric": METRIC,
@@ -383,18 +422,24 @@ def main():
             "mean_attempts_per_propagation": float(np.mean(attempts)) / B,
         },
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "bound": "fp64",
+            "achieved": achieved_tflops,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": traffic,
-            "accounting": ("achieved counts the reference's fft_counter bytes, 72 N-point FFTs x 32 N B per "
-                           "attempted step; the kernel runs 24 N-point transforms and 11 Raman convolutions "
-                           "through N/2-point transform pairs per attempted step, none of them through HBM, so "
-                           "frac can exceed 1; traffic is the measured HBM bytes of the launch"),
+            "hbm_GBps": None if traffic is None else traffic / mean_kernel_s / 1e9,
+            "hbm_frac": None if traffic is None else traffic / mean_kernel_s / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_GBps": fft_bytes_per_launch / mean_kernel_s / 1e9,
+            "accounting": ("achieved: the reference's fft_counter work, 72 N-point FFTs of 5 N log2 N flops per "
+                           "attempted step, over the kernel's HIP-event time, against the FP64 peak (the kernel "
+                           "keeps every transform in registers and LDS, so FP64 issue and LDS-exchange latency "
+                           "bind it, not HBM); traffic: measured HBM bytes of the launch (hbm_GBps, hbm_frac "
+                           "against 8 TB/s); algorithmic_GBps: the same fft_counter work as bytes (32 N per "
+                           "FFT), which exceeds the HBM peak because no transform goes through HBM"),
         },
         "cpu_baseline": None,
+        "error": None,
     }
     if not args.no_ensemble:
         line["ensemble"] = ensemble_measurement()
@@ -405,8 +450,9 @@ def main():
     if not args.no_surrogate:
         line["surrogate"] = surrogate_measurement(args.surrogate_train, args.surrogate_steps, args.surrogate_length)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(inputs[0], args.length, float(np.mean(attempts)) / B, args.cpu_seconds,
-                                            min(os.cpu_count() or 1, args.cpu_workers))
+        line["cpu_baseline"], line["error"] = cpu_baseline(
+            inputs[0], args.length, out[0], res[0]["n_attempts"], attempts_launch / B, args.cpu_seconds,
+            min(os.cpu_count() or 1, args.cpu_workers))
     print(json.dumps(line))
 
 
