@@ -18,10 +18,12 @@ const LaunchTable* table_2048();
 const LaunchTable* table_4096();
 const LaunchTable* table_8192();
 const LaunchTable* table_16384();
+const LaunchTable* table_32768();
+const LaunchTable* table_65536();
 
 const LaunchTable* find_table(long long n) {
-  const LaunchTable* all[] = {table_256(), table_512(), table_1024(), table_2048(), table_4096(), table_8192(),
-                               table_16384()};
+  const LaunchTable* all[] = {table_256(),  table_512(),   table_1024(),  table_2048(), table_4096(),
+                               table_8192(), table_16384(), table_32768(), table_65536()};
   for (const LaunchTable* t : all)
     if (t->n == n) return t;
   return nullptr;
@@ -223,7 +225,7 @@ int gnlse_set_linear(gnlse_ctx* c, const double* factor) {
   RET_IF(check_ready(c, false, false));
   if (!factor) return fail(GNLSE_ERR_ARG, "null factor");
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(upload(c, c->lin, factor, 2 * c->n * sizeof(double)));
+  RET_IF(upload_spec(c, c->lin, factor, 2));
   c->have_lin = true;
   c->have_lin2 = false;
   c->efix_h = NAN;
@@ -240,9 +242,9 @@ int gnlse_set_nonlinear(gnlse_ctx* c, int kind, double gamma, double f_R, double
     return fail(GNLSE_ERR_ARG, "nl_mult and h_R are required for NL_ALL");
   HIPCHK(hipSetDevice(c->device));
   const size_t n = (size_t)c->n;
-  if (omega) RET_IF(upload(c, c->omega, omega, n * sizeof(double)));
-  if (nl_mult) RET_IF(upload(c, c->nlmul, nl_mult, 2 * n * sizeof(double)));
-  if (h_R) RET_IF(upload(c, c->hr, h_R, 2 * n * sizeof(double)));
+  if (omega) RET_IF(upload_spec(c, c->omega, omega, 1));
+  if (nl_mult) RET_IF(upload_spec(c, c->nlmul, nl_mult, 2));
+  if (h_R) RET_IF(upload_spec(c, c->hr, h_R, 2));
   c->ops.nl_kind = kind;
   c->ops.wdm = 0;  // single-channel fields again
   c->ops.fac = make_double2(0.0, gamma);
@@ -266,7 +268,7 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
   if (c->ops.wdm) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields carry no gain model");
   if (!(E_sat > 0.0) || !(length > 0.0)) return fail(GNLSE_ERR_ARG, "E_sat and length must be positive");
   const size_t n = (size_t)c->n;
-  RET_IF(upload(c, c->gprof, profile, n * sizeof(double)));
+  RET_IF(upload_spec(c, c->gprof, profile, 1));
   RET_IF(upload(c, c->simpw, simpson_w, n * sizeof(double)));
   c->ops.gain = GAIN_SATURABLE;
   c->ops.gain_log = gain_log;
@@ -298,6 +300,7 @@ int gnlse_set_gain(gnlse_ctx* c, int enabled, double gain_log, double length, do
 int gnlse_set_linear_wdm(gnlse_ctx* c, const double* factor0, const double* factor1) {
   RET_IF(check_ready(c, false, false));
   if (!factor0 || !factor1) return fail(GNLSE_ERR_ARG, "null factor");
+  if (c->tab->blocks > 1) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields run on grids of up to 16384 samples");
   HIPCHK(hipSetDevice(c->device));
   RET_IF(upload(c, c->lin, factor0, 2 * (size_t)c->n * sizeof(double)));
   RET_IF(upload(c, c->lin2, factor1, 2 * (size_t)c->n * sizeof(double)));
@@ -311,6 +314,7 @@ int gnlse_set_linear_wdm(gnlse_ctx* c, const double* factor0, const double* fact
 
 int gnlse_set_nonlinear_wdm(gnlse_ctx* c, double gamma0, double gamma1) {
   RET_IF(check_ready(c, false, false));
+  if (c->tab->blocks > 1) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields run on grids of up to 16384 samples");
   if (c->ops.gain != GAIN_NONE) return fail(GNLSE_ERR_UNSUPPORTED, "WDM fields carry no gain model");
   c->ops.wdm = 1;
   c->ops.nl_kind = NL_KERR;
@@ -334,10 +338,10 @@ int gnlse_set_yb_gain(gnlse_ctx* c, const double* ratio_over_psat, const double*
     for (size_t i = 0; i < n; ++i)
       if (!std::isfinite(a[i])) return fail(GNLSE_ERR_ARG, "Yb arrays must be finite (zero where a cross section vanishes)");
   HIPCHK(hipSetDevice(c->device));
-  RET_IF(upload(c, c->yb_rs, ratio_over_psat, n * sizeof(double)));
-  RET_IF(upload(c, c->yb_ip, inv_psat, n * sizeof(double)));
-  RET_IF(upload(c, c->yb_as, alpha_s, n * sizeof(double)));
-  RET_IF(upload(c, c->yb_es, eta_s, n * sizeof(double)));
+  RET_IF(upload_spec(c, c->yb_rs, ratio_over_psat, 1));
+  RET_IF(upload_spec(c, c->yb_ip, inv_psat, 1));
+  RET_IF(upload_spec(c, c->yb_as, alpha_s, 1));
+  RET_IF(upload_spec(c, c->yb_es, eta_s, 1));
   c->ops.gain = GAIN_YB;
   c->ops.yb_pfac = power_factor;
   c->ops.yb_rp = ratio_p;
@@ -375,8 +379,10 @@ int gnlse_fft(gnlse_ctx* c, int sign, int64_t batch, const double* in, double* o
   HIPCHK(hipSetDevice(c->device));
   RET_IF(io_prepare(c, batch, in, c->io_in));
   RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure_fft_ws(c, batch));
   RET_IF(timed_begin(c));
-  RET_IF(timed_end(c, c->tab->fft(c->stream, c->ops, sign, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p)));
+  RET_IF(timed_end(c, c->tab->fft(c->stream, c->ops, sign, batch, (const double2*)c->io_in.p, (double2*)c->io_out.p,
+                                  (double2*)c->ws.p)));
   return io_finish(c, batch, out);
 }
 
@@ -650,7 +656,7 @@ int gnlse_propagate_adaptive_device_noalloc(gnlse_ctx* c, int method, int64_t ba
   if (batch <= 0 || batch > 0x7fffffff || !d_A_in || !d_A_out || !d_results)
     return fail(GNLSE_ERR_ARG, "invalid arguments");
   RET_IF(check_pump(c, batch));
-  const size_t per = (size_t)fields_per_instance(c) * NSLOT * c->n * sizeof(double2);
+  const size_t per = (size_t)fields_per_instance(c) * c->tab->nslot * c->n * sizeof(double2);
   if ((size_t)batch * per > (size_t)GNLSE_NOALLOC_SCRATCH_BYTES)
     return fail(GNLSE_ERR_ARG, "batch " + std::to_string(batch) + " needs " + std::to_string((size_t)batch * per) +
                                    " bytes of workspace; the no-allocation entry point holds " +
@@ -682,11 +688,14 @@ int gnlse_trace_characteristics(gnlse_ctx* c, int64_t count, const double* A, co
   RET_IF(ensure(c->ch_w, (size_t)c->n * sizeof(double)));
   HIPCHK(hipMemcpyAsync(c->ch_w.p, simpson_w, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   // pyofss spectra (field.fft), then the characteristics of every trace
-  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p);
+  // (the large grids' transform workspace then holds their powers and flags)
+  RET_IF(ensure_fft_ws(c, count));
+  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p,
+                             (double2*)c->ws.p);
   if (e == hipSuccess)
     e = c->tab->characts(c->stream, count, (const double2*)c->io_in.p, (const double2*)c->io_b.p,
                          (const double*)c->ch_w.p, (double*)c->ch_out.p, (int*)c->ch_npk.p, (int*)c->ch_pk.p,
-                         max_peaks);
+                         max_peaks, (double2*)c->ws.p);
   if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("characteristics: ") + hipGetErrorString(e));
   HIPCHK(hipMemcpyAsync(out, c->ch_out.p, (size_t)count * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(n_peaks, c->ch_npk.p, (size_t)count * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -709,7 +718,9 @@ int gnlse_trace_powers(gnlse_ctx* c, int64_t count, const double* A, int64_t n_o
   RET_IF(io_prepare(c, count, A, c->io_in));
   RET_IF(ensure(c->io_b, fb));
   RET_IF(ensure(c->ch_out, 2 * ob));
-  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p);
+  RET_IF(ensure_fft_ws(c, count));
+  hipError_t e = c->tab->fft(c->stream, c->ops, +1, count, (const double2*)c->io_in.p, (double2*)c->io_b.p,
+                             (double2*)c->ws.p);
   if (e == hipSuccess) {
     const dim3 grid((unsigned)count, (unsigned)((n_out + kPowT - 1) / kPowT));
     double* out = (double*)c->ch_out.p;

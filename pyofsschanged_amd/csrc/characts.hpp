@@ -144,14 +144,13 @@ __device__ __forceinline__ double char_prominence_par(CharScratch& s, const doub
   return h - fmax(lmin, rmin);
 }
 
+// The characteristics of trace b, with its powers in x[N] and the peak flags
+// in flag[N] (LDS, or global scratch on the large grids).
 template <int N>
-__global__ void __launch_bounds__(kCharT) k_characts(const double2* __restrict__ A, const double2* __restrict__ S,
-                                                    const double* __restrict__ w, double* __restrict__ out,
-                                                    int* __restrict__ npk, int* __restrict__ pk, int max_pk) {
-  __shared__ double x[N];
-  __shared__ unsigned char flag[N];
-  __shared__ CharScratch s;
-  __shared__ int cnt[kCharT + 1];
+__device__ __forceinline__ void characts_body(const double2* __restrict__ A, const double2* __restrict__ S,
+                                              const double* __restrict__ w, double* __restrict__ out,
+                                              int* __restrict__ npk, int* __restrict__ pk, int max_pk, double* x,
+                                              unsigned char* flag, CharScratch& s, int* cnt) {
   const long long b = blockIdx.x;
   const int tid = threadIdx.x;
 
@@ -321,6 +320,30 @@ __global__ void __launch_bounds__(kCharT) k_characts(const double2* __restrict__
     out[b * 4 + 2] = duration;
     out[b * 4 + 3] = spec;
   }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kCharT) k_characts(const double2* __restrict__ A, const double2* __restrict__ S,
+                                                    const double* __restrict__ w, double* __restrict__ out,
+                                                    int* __restrict__ npk, int* __restrict__ pk, int max_pk) {
+  __shared__ double x[N];
+  __shared__ unsigned char flag[N];
+  __shared__ CharScratch s;
+  __shared__ int cnt[kCharT + 1];
+  characts_body<N>(A, S, w, out, npk, pk, max_pk, x, flag, s, cnt);
+}
+
+// Large grids: the powers and flags of trace b in scratch + b (N + N/8)
+// doubles (written and read by this workgroup only; its barriers order them).
+template <int N>
+__global__ void __launch_bounds__(kCharT) k_characts_global(const double2* __restrict__ A,
+                                                           const double2* __restrict__ S, const double* __restrict__ w,
+                                                           double* __restrict__ out, int* __restrict__ npk,
+                                                           int* __restrict__ pk, int max_pk, double2* scratch) {
+  __shared__ CharScratch s;
+  __shared__ int cnt[kCharT + 1];
+  double* x = reinterpret_cast<double*>(scratch) + (size_t)blockIdx.x * (N + N / 8);
+  characts_body<N>(A, S, w, out, npk, pk, max_pk, x, reinterpret_cast<unsigned char*>(x + N), s, cnt);
 }
 
 }  // namespace gn

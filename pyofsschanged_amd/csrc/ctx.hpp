@@ -144,6 +144,26 @@ inline int check_ready(gnlse_ctx* c, bool need_lin, bool need_nl) {
 }
 
 inline int ensure_ws(gnlse_ctx* c, int64_t batch) {
-  return ensure(c->ws, (size_t)batch * fields_per_instance(c) * NSLOT * c->n * sizeof(double2));
+  return ensure(c->ws, (size_t)batch * fields_per_instance(c) * c->tab->nslot * c->n * sizeof(double2));
+}
+// Workspace of the standalone transform (one field per instance; the large
+// grids use it, and their trace characteristics reuse it afterwards).
+inline int ensure_fft_ws(gnlse_ctx* c, int64_t batch) {
+  return ensure(c->ws, (size_t)batch * c->n * sizeof(double2));
+}
+
+// Upload an array indexed by frequency (FFT order) with `comp` doubles per
+// element, in the kernels' spectral layout: FFT order, or on the large grids
+// (tab->blocks = R > 1) block layout, position k2 (n/R) + k1 holding index
+// R k1 + k2.
+inline int upload_spec(gnlse_ctx* c, DevBuf& b, const double* src, int comp) {
+  const int64_t n = c->n, R = c->tab->blocks;
+  if (R <= 1) return upload(c, b, src, (size_t)n * comp * sizeof(double));
+  const int64_t M = n / R;
+  std::vector<double> perm((size_t)n * comp);
+  for (int64_t k2 = 0; k2 < R; ++k2)
+    for (int64_t k1 = 0; k1 < M; ++k1)
+      for (int q = 0; q < comp; ++q) perm[(size_t)(k2 * M + k1) * comp + q] = src[(size_t)(R * k1 + k2) * comp + q];
+  return upload(c, b, perm.data(), perm.size() * sizeof(double));
 }
 

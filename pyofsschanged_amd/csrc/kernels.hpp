@@ -41,7 +41,15 @@ struct AdaptiveParams {
 
 struct LaunchTable {
   int n, t;
-  hipError_t (*fft)(hipStream_t, const Ops&, int sign, long long batch, const double2* in, double2* out);
+  int nslot;  // workspace slots (N complex) per field of an instance
+  // spectral layout of the operator arrays and of every spectrum inside the
+  // kernels: 1 = FFT order; R > 1 = R blocks, position k2 (N/R) + k1 holding
+  // frequency index R k1 + k2 (the large grids of big.hpp)
+  int blocks;
+  // gnlse_fft: natural (FFT) order in and out; ws holds one field per
+  // instance (used by the large grids only)
+  hipError_t (*fft)(hipStream_t, const Ops&, int sign, long long batch, const double2* in, double2* out,
+                    double2* ws);
   hipError_t (*make_exp)(hipStream_t, const Ops&, double h, double2* out);
   hipError_t (*linear)(hipStream_t, const Ops&, double h, long long batch, const double2* in, double2* out,
                        double2* ws);
@@ -55,8 +63,10 @@ struct LaunchTable {
                       double2* out, double2* ws);
   hipError_t (*adaptive)(hipStream_t, const Ops&, int method, const AdaptiveParams&, long long batch,
                          const double2* in, double2* out, double2* ws);
+  // scratch: count x (N doubles + N bytes), the large grids' powers and
+  // peak flags (the other grids hold them in LDS)
   hipError_t (*characts)(hipStream_t, long long count, const double2* A, const double2* S, const double* w,
-                         double* out, int* npk, int* pk, int max_pk);
+                         double* out, int* npk, int* pk, int max_pk, double2* scratch);
 };
 
 const LaunchTable* find_table(long long n);
@@ -424,7 +434,8 @@ namespace gn {
 
 template <int N, int T>
 struct Launcher {
-  static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out) {
+  static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out,
+                        double2*) {
     hipLaunchKernelGGL((k_fft<N, T>), dim3(batch), dim3(T), 0, s, o, sign, in, out);
     return hipGetLastError();
   }
@@ -502,12 +513,12 @@ struct Launcher {
     return hipGetLastError();
   }
   static hipError_t characts(hipStream_t s, long long count, const double2* A, const double2* S, const double* w,
-                             double* out, int* npk, int* pk, int max_pk) {
+                             double* out, int* npk, int* pk, int max_pk, double2*) {
     hipLaunchKernelGGL((k_characts<N>), dim3((unsigned)count), dim3(kCharT), 0, s, A, S, w, out, npk, pk, max_pk);
     return hipGetLastError();
   }
-  static constexpr LaunchTable table = {N, T, &fft, &make_exp, &linear, &linfac, &nonlin, &exp_nonlin, &step, &fixed,
-                                        &adaptive, &characts};
+  static constexpr LaunchTable table = {N,       T,     NSLOT, 1,           &fft,  &make_exp, &linear,  &linfac,
+                                        &nonlin, &exp_nonlin, &step, &fixed, &adaptive, &characts};
 };
 
 }  // namespace gn

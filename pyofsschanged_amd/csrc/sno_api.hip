@@ -422,6 +422,7 @@ int gnlse_sno_encode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const double* d_
     return fail(GNLSE_ERR_ARG, "invalid encode arguments");
   RET_IF(check_ready(c, false, false));
   if (c->n != s->N) return fail(GNLSE_ERR_ARG, "context grid differs from the surrogate grid");
+  if (c->tab->blocks > 1) return fail(GNLSE_ERR_UNSUPPORTED, "surrogate grids have up to 16384 samples");
   HIPCHK(hipSetDevice(s->device));
   RET_IF(ensure(c->io_b, (size_t)batch * s->N * sizeof(double2)));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -430,17 +431,17 @@ int gnlse_sno_encode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const double* d_
   hipError_t e = hipSuccess;
   if (!time_domain) {
     // pyofss fft, then the natural-order encoding (phase in natural order)
-    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp);
+    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp, nullptr);
     if (e == hipSuccess)
       hipLaunchKernelGGL(k_sno_encode, dim3(nb), dim3(256), 0, c->stream, (const double2*)tmp,
                          (const double2*)d_phase, scale, d_out, (long long)batch, s->N, channels);
   } else if (d_phase) {
     // ifft(phase * fft(A)) (phase in FFT order), then the time encoding
-    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp);
+    e = c->tab->fft(c->stream, c->ops, +1, batch, (const double2*)d_A, tmp, nullptr);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_sno_cmul, dim3(nb), dim3(256), 0, c->stream, tmp, (const double2*)d_phase, (long long)batch,
                          s->N);
-      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, tmp);
+      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, tmp, nullptr);
     }
     if (e == hipSuccess)
       hipLaunchKernelGGL(k_sno_pack_time, dim3(nb), dim3(256), 0, c->stream, (const double2*)tmp, scale, d_out,
@@ -460,6 +461,7 @@ int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y
   if (!s || !d_y || !d_A || batch < 1 || !(scale > 0.0)) return fail(GNLSE_ERR_ARG, "invalid decode arguments");
   RET_IF(check_ready(c, false, false));
   if (c->n != s->N) return fail(GNLSE_ERR_ARG, "context grid differs from the surrogate grid");
+  if (c->tab->blocks > 1) return fail(GNLSE_ERR_UNSUPPORTED, "surrogate grids have up to 16384 samples");
   HIPCHK(hipSetDevice(s->device));
   RET_IF(ensure(c->io_b, (size_t)batch * s->N * sizeof(double2)));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -469,14 +471,14 @@ int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y
   if (!time_domain) {
     hipLaunchKernelGGL(k_sno_decode, dim3(nb), dim3(256), 0, c->stream, d_y, (const double2*)d_phase, scale, tmp,
                        (long long)batch, s->N);
-    e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A);
+    e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A, nullptr);
   } else if (d_phase) {
     hipLaunchKernelGGL(k_sno_unpack_time, dim3(nb), dim3(256), 0, c->stream, d_y, scale, tmp, (long long)batch * s->N);
-    e = c->tab->fft(c->stream, c->ops, +1, batch, tmp, tmp);
+    e = c->tab->fft(c->stream, c->ops, +1, batch, tmp, tmp, nullptr);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_sno_cmul, dim3(nb), dim3(256), 0, c->stream, tmp, (const double2*)d_phase, (long long)batch,
                          s->N);
-      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A);
+      e = c->tab->fft(c->stream, c->ops, -1, batch, tmp, (double2*)d_A, nullptr);
     }
   } else {
     hipLaunchKernelGGL(k_sno_unpack_time, dim3(nb), dim3(256), 0, c->stream, d_y, scale, (double2*)d_A,

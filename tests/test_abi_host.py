@@ -35,7 +35,8 @@ def test_library_exports_every_declared_function():
     for n in (256, 512, 1024, 2048, 4096, 8192, 16384):
         assert _lib.grid_supported(n)
     assert not _lib.grid_supported(1000)
-    assert not _lib.grid_supported(32768)
+    assert _lib.grid_supported(32768) and _lib.grid_supported(65536)
+    assert not _lib.grid_supported(131072)
 
 
 def test_struct_layouts_match_header():
