@@ -332,6 +332,16 @@ int gnlse_gemm_bf16(int device, int at, int bt, int64_t M, int64_t N, int64_t K,
                     const void* d_B, int64_t ldb, float* d_C, int64_t ldc, int64_t batch, int64_t sA, int64_t sB,
                     int64_t sC, int ksplit);
 
+/* Self-test of the engine's workgroup reductions (no counterpart in the
+ * reference): for `blocks` workgroups of `threads` threads (64, 128, 256, 512
+ * or 1024), each thread's m values (2, 3, 4, 9 or 18) from
+ * in[(b m + j) threads + t] are summed over the workgroup once as m sums at
+ * a time (wg_sum_n, the routine of the gain stages and the error norm) into
+ * out_n and once one by one (wg_sum) into out_1, same layout; the two must
+ * agree bit for bit in every thread. Host arrays of blocks m threads doubles. */
+int gnlse_check_reductions(int device, int threads, int m, int64_t blocks, const double* in, double* out_n,
+                           double* out_1);
+
 #ifdef __cplusplus
 }
 #endif

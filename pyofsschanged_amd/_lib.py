@@ -133,6 +133,7 @@ SIGNATURES = {
     "gnlse_sno_encode": [_P, _P, _I64, _P, _D, _P, _INT, _INT, _P],
     "gnlse_sno_decode": [_P, _P, _I64, _P, _D, _P, _INT, _P],
     "gnlse_sno_energy_defect": [_P, _I64, _P, _P, _P],
+    "gnlse_check_reductions": [_INT, _INT, _INT, _I64, _P, _P, _P],
     "gnlse_gemm_bf16": [_INT, _INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _I64, _INT],
 }
 

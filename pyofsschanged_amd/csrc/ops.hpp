@@ -201,6 +201,46 @@ __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2
   }
 }
 
+// Sum of v over a workgroup of NW wavefronts through red (NW doubles of
+// LDS); every thread gets the same value: a butterfly over the 64 lanes, then
+// the wavefront partial sums in wavefront order.
+template <int NW>
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) s += red[w];
+  return s;
+}
+// M sums at once through buf (M NW doubles of LDS), each in the order of
+// wg_sum, so the results are those of M wg_sum calls, bit for bit, with two
+// barriers instead of 2 M.
+template <int NW, int M>
+__device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) v[m] += __shfl_xor(v[m], o, 64);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) buf[m * NW + (threadIdx.x >> 6)] = v[m];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += buf[m * NW + w];
+    v[m] = s;
+  }
+}
+
 // KIND fixes the nonlinear operator at compile time (NL_KERR, NL_ALL) or
 // leaves it to ops.nl_kind (NL_GENERIC); a fixed kind lets the compiler drop
 // the other operators and their register live ranges.
@@ -302,40 +342,13 @@ struct Dev {
   }
 
   // Sum over the workgroup; every thread gets the same value (fixed order).
-  __device__ __forceinline__ double block_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) s += red[w];
-    return s;
-  }
+  __device__ __forceinline__ double block_sum(double v) { return wg_sum<NW>(v, red); }
 
   // M block sums at once through buf (M * NW doubles of LDS, not in use);
   // each value is reduced in the order of block_sum.
   template <int M>
   __device__ __forceinline__ void block_sum_n(double (&v)[M], double* buf) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m] += __shfl_xor(v[m], o, 64);
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-      for (int m = 0; m < M; ++m) buf[m * NW + (threadIdx.x >> 6)] = v[m];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      double s = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) s += buf[m * NW + w];
-      v[m] = s;
-    }
+    wg_sum_n<NW, M>(v, buf);
   }
 
   // Pump power of instance b (the Yb model only): read before the first
