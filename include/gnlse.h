@@ -37,8 +37,9 @@ enum {
   GNLSE_ERR_STATE = -4
 };
 
-/* integrators, pyofss/modules/solver.py:339-354 and :266-275 */
-enum { GNLSE_METHOD_RK4IP = 0, GNLSE_METHOD_SS_SYMMETRIC = 1 };
+/* integrators, pyofss/modules/solver.py:339-354, :266-275 and :330-337
+ * (ss_sym_rk4, also opencl_fibre.py:929-970 cl_ss_sym_rk4) */
+enum { GNLSE_METHOD_RK4IP = 0, GNLSE_METHOD_SS_SYMMETRIC = 1, GNLSE_METHOD_SS_SYM_RK4 = 2 };
 
 /* nonlinear operator variants, pyofss/modules/nonlinearity.py:131-152 */
 enum {

@@ -11,7 +11,7 @@ import numpy as np  # noqa: F401  (pyofss re-exports numpy the same way)
 from .domain import (Domain, align_with_nu_grid, dlambda_to_dnu, dnu_to_dlambda, lambda_to_nu, lambda_to_omega,
                      nu_to_lambda, nu_to_omega, omega_to_lambda, omega_to_nu)
 from .field import (chirp, energy, fft, fftshift, get_downsampled, get_duration, get_duration_spec, ifft, ifftshift,
-                    inst_freq, loss_infrared_db, loss_rayleigh_db, phase, spectral_power, spectrum_width_params,
+                    loss_infrared_db, loss_rayleigh_db, phase, spectral_power, spectrum_width_params,
                     temporal_power)
 from .modules.amplifier import Amplifier, Amplifier2LevelModel
 from .modules.fibre import Fibre

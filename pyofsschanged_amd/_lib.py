@@ -17,6 +17,7 @@ LIB_PATH = os.environ.get("GNLSE_LIBRARY") or os.path.join(os.path.dirname(os.pa
 
 METHOD_RK4IP = 0
 METHOD_SS_SYMMETRIC = 1
+METHOD_SS_SYM_RK4 = 2
 
 NL_KERR = 0
 NL_ALL = 1

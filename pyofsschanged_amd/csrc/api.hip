@@ -89,7 +89,9 @@ static int io_finish(gnlse_ctx* c, int64_t batch, double* out) {
   return GNLSE_OK;
 }
 
-static bool method_ok(int m) { return m == GNLSE_METHOD_RK4IP || m == GNLSE_METHOD_SS_SYMMETRIC; }
+static bool method_ok(int m) {
+  return m == GNLSE_METHOD_RK4IP || m == GNLSE_METHOD_SS_SYMMETRIC || m == GNLSE_METHOD_SS_SYM_RK4;
+}
 
 // Down-sampled powers of traces (one workgroup row per trace, a thread per
 // output point). A: fields; S: their pyofss spectra fft(A) in FFT order.
@@ -834,6 +836,8 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
     return fail(GNLSE_ERR_ARG, "an ensemble has 1 to 4 groups");
   AdaptiveParams prm;
   RET_IF(adaptive_params(method, p, prm));
+  if (method == GNLSE_METHOD_SS_SYM_RK4)
+    return fail(GNLSE_ERR_UNSUPPORTED, "ensembles run rk4ip or ss_symmetric");
   for (int i = 0; i < ngroups; ++i) {
     RET_IF(check_ready(ctxs[i], true, true));
     const int64_t n = ctxs[i]->n;

@@ -518,11 +518,54 @@ __device__ __forceinline__ void big_ss_symmetric(BigDev<N, YB>& d, const double2
   d.linear(sU, out, E, 0.5 * h);
 }
 
+// solver.py:330-337 ss_sym_rk4 (opencl_fibre.py:929-970 cl_ss_sym_rk4): L(h/2),
+// the classical rk4 on the nonlinear operator, L(h/2). in -> out (out may
+// be in).
+template <int N, bool YB>
+__device__ __forceinline__ void big_ss_sym_rk4(BigDev<N, YB>& d, const double2* in, double2* out, const double2* E,
+                                               double h) {
+  using G = BigGeo<N>;
+  double2* sAL = d.slot(B_AI);
+  double2* sK0 = d.slot(B_K0);
+  double2* sK1 = d.slot(B_K1);
+  double2* sU = d.slot(B_U);
+  d.linear(in, sAL, E, 0.5 * h);
+  d.nonlin(sAL, sK0, h);  // k0
+#pragma unroll 4
+  for (int q = 0; q < G::PT; ++q) {
+    const unsigned p = d.pos(q);
+    sU[p] = cadd(sAL[p], cscale(sK0[p], 0.5));
+  }
+  d.nonlin(sU, sK1, h);  // k1
+#pragma unroll 4
+  for (int q = 0; q < G::PT; ++q) {
+    const unsigned p = d.pos(q);
+    sU[p] = cadd(sAL[p], cscale(sK1[p], 0.5));
+  }
+  d.nonlin(sU, sU, h);  // k2
+#pragma unroll 4
+  for (int q = 0; q < G::PT; ++q) {
+    const unsigned p = d.pos(q);
+    const double2 k2 = sU[p];
+    sK1[p] = cadd(sK1[p], k2);  // k1 + k2
+    sU[p] = cadd(sAL[p], k2);   // A_L + k2
+  }
+  d.nonlin(sU, sU, h);  // k3
+#pragma unroll 4
+  for (int q = 0; q < G::PT; ++q) {
+    const unsigned p = d.pos(q);
+    const double2 s = cadd(cadd(sK0[p], cscale(sK1[p], 2.0)), sU[p]);
+    sU[p] = cadd(sAL[p], make_double2(s.x / 6.0, s.y / 6.0));
+  }
+  d.linear(sU, out, E, 0.5 * h);
+}
+
 template <int METHOD, int N, bool YB>
 __device__ __forceinline__ void big_integrator(BigDev<N, YB>& d, const double2* in, double2* out, const double2* E,
                                                double h, double2* nA = nullptr, bool nA_ready = false,
                                                bool nA_store = false) {
   if constexpr (METHOD == M_RK4IP) big_rk4ip(d, in, out, E, h, nA, nA_ready, nA_store);
+  else if constexpr (METHOD == M_SS_SYM_RK4) big_ss_sym_rk4(d, in, out, E, h);
   else big_ss_symmetric(d, in, out, E, h);
 }
 
@@ -789,6 +832,9 @@ struct BigLauncher {
     if (o.wdm) return hipErrorNotSupported;                                                       \
     if (method == M_RK4IP && o.gain == GAIN_YB) hipLaunchKernelGGL((KERNEL<N, M_RK4IP, true>), __VA_ARGS__);  \
     else if (method == M_RK4IP) hipLaunchKernelGGL((KERNEL<N, M_RK4IP, false>), __VA_ARGS__);     \
+    else if (method == M_SS_SYM_RK4 && o.gain == GAIN_YB)                                          \
+      hipLaunchKernelGGL((KERNEL<N, M_SS_SYM_RK4, true>), __VA_ARGS__);                            \
+    else if (method == M_SS_SYM_RK4) hipLaunchKernelGGL((KERNEL<N, M_SS_SYM_RK4, false>), __VA_ARGS__); \
     else if (o.gain == GAIN_YB) hipLaunchKernelGGL((KERNEL<N, M_SS_SYMMETRIC, true>), __VA_ARGS__); \
     else hipLaunchKernelGGL((KERNEL<N, M_SS_SYMMETRIC, false>), __VA_ARGS__);                    \
   } while (0)

@@ -469,6 +469,16 @@ struct Launcher {
   }
   static hipError_t step(hipStream_t s, const Ops& o, int method, double h, double h_exp, long long batch, const double2* in,
                          double2* out, double2* ws) {
+    if (method == M_SS_SYM_RK4) {
+      if (o.wdm) return hipErrorNotSupported;
+      if (o.gain == GAIN_YB)
+        hipLaunchKernelGGL((k_step<N, T, M_SS_SYM_RK4, NL_GENERIC, true>), dim3(batch), dim3(T), 0, s, o, h, h_exp, in,
+                           out, ws);
+      else
+        hipLaunchKernelGGL((k_step<N, T, M_SS_SYM_RK4, NL_GENERIC, false>), dim3(batch), dim3(T), 0, s, o, h, h_exp, in,
+                           out, ws);
+      return hipGetLastError();
+    }
     if (o.wdm && method == M_RK4IP)
       hipLaunchKernelGGL((k_step_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, h, h_exp, in, out, ws);
     else if (o.wdm)
@@ -479,6 +489,15 @@ struct Launcher {
   }
   static hipError_t fixed(hipStream_t s, const Ops& o, int method, const FixedParams& p, long long batch,
                           const double2* in, double2* out, double2* ws) {
+    if (method == M_SS_SYM_RK4) {
+      if (o.wdm) return hipErrorNotSupported;
+      if (o.gain == GAIN_YB)
+        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, true>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      else
+        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, false>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+                           ws);
+      return hipGetLastError();
+    }
     if (o.wdm && method == M_RK4IP)
       hipLaunchKernelGGL((k_fixed_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.wdm)
@@ -496,6 +515,16 @@ struct Launcher {
   }
   static hipError_t adaptive(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
                              const double2* in, double2* out, double2* ws) {
+    if (method == M_SS_SYM_RK4) {
+      if (o.wdm) return hipErrorNotSupported;
+      if (o.gain == GAIN_YB)
+        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, true>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+                           ws);
+      else
+        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, false>), dim3(batch), dim3(T), 0, s, o, p, in,
+                           out, ws);
+      return hipGetLastError();
+    }
     if (o.wdm && method == M_RK4IP)
       hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_RK4IP>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.wdm)

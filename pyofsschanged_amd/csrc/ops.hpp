@@ -21,7 +21,7 @@
 namespace gn {
 
 enum NlKind { NL_GENERIC = -1, NL_KERR = 0, NL_ALL = 1, NL_SS = 2, NL_RS = 3, NL_SS_RS = 4 };
-enum Method { M_RK4IP = 0, M_SS_SYMMETRIC = 1 };
+enum Method { M_RK4IP = 0, M_SS_SYMMETRIC = 1, M_SS_SYM_RK4 = 2 };
 // spectral multiplier kinds
 enum MulKind { MUL_ARRAY = 0, MUL_OMEGA = 1, MUL_IOMEGA = 2, MUL_GAIN = 3, MUL_YB = 4 };
 // gain models coupled into the linear operator
@@ -1014,11 +1014,56 @@ __device__ __forceinline__ void ss_symmetric(Dev<N, T, K, YB>& d, double2 (&x)[N
   }
 }
 
+// solver.py:330-337 ss_sym_rk4: L(h/2), the classical rk4 of solver.py
+// (k_j = h N(A_L + c_j k_(j-1)), A_N = A_L + (k0 + 2 (k1 + k2) + k3)/6) on the
+// nonlinear operator alone, L(h/2); opencl_fibre.py:929-970 cl_ss_sym_rk4 is
+// the same scheme. One nonlinear call site in a 4-stage loop.
+template <int N, int T, int K, bool YB>
+__device__ __forceinline__ void ss_sym_rk4(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in,
+                                           const double2* E, double h, double2* ws) {
+  constexpr int P = N / T;
+  double2* sAL = ws + S_AI * N;
+  double2* sK0 = ws + S_K0 * N;
+  double2* sK1 = ws + S_K1 * N;
+  double2* sT = ws + S_T * N;
+  double2* sU = ws + S_U * N;
+  const double hh = 0.5 * h;
+  d.load(x, in);
+  d.linear(x, E, hh);
+  d.store(sAL, x);
+#pragma unroll 1
+  for (int st = 0; st < 4; ++st) {
+    if (st > 0) {  // A_L + c k_(st-1), c = 1/2, 1/2, 1
+      const double c = st == 3 ? 1.0 : 0.5;
+#pragma unroll
+      for (int k = 0; k < P; ++k) x[k] = cadd(sAL[d.idx(k)], cscale(x[k], c));
+    }
+    d.nonlin(x, sU, sT);
+#pragma unroll
+    for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
+    if (st == 0) d.store(sK0, x);
+    else if (st == 1) d.store(sK1, x);
+    else if (st == 2) {  // k1 + k2 kept; x stays k2 for the next stage
+#pragma unroll
+      for (int k = 0; k < P; ++k) sK1[d.idx(k)] = cadd(sK1[d.idx(k)], x[k]);
+    } else {  // A_N = A_L + (k0 + 2 (k1 + k2) + k3) / 6
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int i = d.idx(k);
+        const double2 s = cadd(cadd(sK0[i], cscale(sK1[i], 2.0)), x[k]);
+        x[k] = cadd(sAL[i], make_double2(s.x / 6.0, s.y / 6.0));
+      }
+    }
+  }
+  d.linear(x, E, hh);
+}
+
 template <int METHOD, int N, int T, int K, bool YB>
 __device__ __forceinline__ void integrator(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E,
                                            double h, double2* ws, double2* nA = nullptr, bool nA_ready = false,
                                            bool nA_store = false) {
   if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);
+  else if constexpr (METHOD == M_SS_SYM_RK4) ss_sym_rk4(d, x, in, E, h, ws);
   else ss_symmetric(d, x, in, E, h, ws);
 }
 

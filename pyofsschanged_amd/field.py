@@ -123,13 +123,9 @@ def energy(A_t, t):
     return float(np.dot(simpson_weights(t * 1e-3), temporal_power(A_t)))
 
 
-def inst_freq(A_t, dt):
-    """field.py:175-184"""
-    return np.append(np.diff(phase(A_t)) / dt, 0.0)
-
-
 def loss_infrared_db(wl):
-    """field.py:187-197"""
+    """field.py:187-197 (examples/fopo_cars/FOPO_model_ocl.py:101 builds the
+    fibre's attenuation from this and loss_rayleigh_db)"""
     p = np.array([-1.75292532e03, 1.95607318e-02])
     return np.exp(p[1] * (wl + p[0]))
 

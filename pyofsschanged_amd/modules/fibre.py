@@ -106,7 +106,12 @@ class Fibre:
         """Advance field.fft_counter as pyofss counts FFTs (field.py:118-134)."""
         nl = _NL_FFTS[self.nonlinearity.kind]
         lin = 2 * self.linearity.channels() + self.linearity.gain_ffts()  # per linear call
-        per_call = 4 * lin + 4 * nl if method == _lib.METHOD_RK4IP else 2 * lin + nl
+        if method == _lib.METHOD_RK4IP:
+            per_call = 4 * lin + 4 * nl
+        elif method == _lib.METHOD_SS_SYM_RK4:  # two linear half steps, four nonlinear evaluations
+            per_call = 2 * lin + 4 * nl
+        else:
+            per_call = 2 * lin + nl
         field.fft_counter += per_call * calls
 
     def prepare(self, domain):

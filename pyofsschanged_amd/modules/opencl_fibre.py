@@ -5,7 +5,7 @@ Notebook code written for the OpenCL fibre (``OpenclFibre(cl_programm=...,
 ...)`` with an ``OpenclProgramm(fast_math=..., use_all=..., dorf=...,
 ctx=..., downsampling=...)``, e.g. examples/notebooks/mamyshev_oscillator_cl.
 ipynb and nonlinear_attractor.ipynb) runs unchanged: the constructors take
-the same arguments, the fixed-step propagation (cl_rk4ip, cl_ss_symmetric)
+the same arguments, the fixed-step propagation (cl_rk4ip, cl_ss_symmetric, cl_ss_sym_rk4)
 runs in the engine's persistent kernel, and every stored trace is reduced on
 the device the way the OpenCL fibre does it: cl_power / cl_fftshift /
 cl_interpolate down-sampled powers, the maximum power, and the energy of
@@ -31,7 +31,8 @@ class FiberInitError(Exception):
     pass
 
 
-_METHODS = {"cl_rk4ip": _lib.METHOD_RK4IP, "cl_ss_symmetric": _lib.METHOD_SS_SYMMETRIC}
+_METHODS = {"cl_rk4ip": _lib.METHOD_RK4IP, "cl_ss_symmetric": _lib.METHOD_SS_SYMMETRIC,
+            "cl_ss_sym_rk4": _lib.METHOD_SS_SYM_RK4}
 
 
 def opencl_simpson_weights(n, dt):
@@ -77,7 +78,7 @@ class OpenclProgramm:
 
 
 class OpenclFibre:
-    """opencl_fibre.py:422-1003 (methods cl_rk4ip and cl_ss_symmetric)."""
+    """opencl_fibre.py:422-1003 (methods cl_rk4ip, cl_ss_symmetric and cl_ss_sym_rk4)."""
 
     def __init__(self, cl_programm, name="cl_fibre", length=1.0, alpha=None, beta=None, gamma=0.0,
                  method="cl_ss_symmetric", total_steps=100, self_steepening=False, centre_omega=None, tau_1=12.2e-3,
@@ -99,8 +100,6 @@ class OpenclFibre:
         self.stepsize = self.length / self.total_steps
         self.zs = np.linspace(0.0, self.length, self.total_steps + 1)
         key = method.lower()
-        if key == "cl_ss_sym_rk4":
-            raise NotImplementedError("cl_ss_sym_rk4 is not supported by the HIP engine (use cl_rk4ip)")
         if key not in _METHODS:
             raise AttributeError(f"OpenclFibre has no method {method!r}")
         self.method = key

@@ -1,7 +1,9 @@
 """Integrators, as pyofss/modules/solver.py:24-354.
 
-The HIP engine implements the two fibre integrators of the stepping path,
-``rk4ip`` (solver.py:339-354) and ``ss_symmetric`` (solver.py:266-275). A
+The HIP engine implements the fibre integrators of the stepping path,
+``rk4ip`` (solver.py:339-354), ``ss_symmetric`` (solver.py:266-275) and
+``ss_sym_rk4`` (solver.py:330-337, the symmetric split step with the
+classical rk4 on the nonlinear operator). A
 call ``Solver(method, f)(A, z, h)`` with a Fibre's FunctionStep runs one
 integrator call on the GPU. The generic ODE methods of the reference (euler,
 rk4, embedded pairs, other split-step variants) take arbitrary Python
@@ -23,7 +25,8 @@ class UnsupportedMethodError(SolverError, NotImplementedError):
     pass
 
 
-_DEVICE_METHODS = {"rk4ip": _lib.METHOD_RK4IP, "ss_symmetric": _lib.METHOD_SS_SYMMETRIC}
+_DEVICE_METHODS = {"rk4ip": _lib.METHOD_RK4IP, "ss_symmetric": _lib.METHOD_SS_SYMMETRIC,
+                   "ss_sym_rk4": _lib.METHOD_SS_SYM_RK4}
 
 
 def device_method(name):
@@ -78,3 +81,10 @@ class Solver:
         fibre = _engine_of(f)
         fibre._count_ffts(_lib.METHOD_SS_SYMMETRIC, 1)
         return fibre._device_context().step(_lib.METHOD_SS_SYMMETRIC, A, h, fibre.linear_step(0.5 * h))
+
+    @staticmethod
+    def ss_sym_rk4(A, z, h, f):
+        """solver.py:330-337 on the GPU."""
+        fibre = _engine_of(f)
+        fibre._count_ffts(_lib.METHOD_SS_SYM_RK4, 1)
+        return fibre._device_context().step(_lib.METHOD_SS_SYM_RK4, A, h, fibre.linear_step(0.5 * h))

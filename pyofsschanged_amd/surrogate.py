@@ -215,7 +215,8 @@ class SegmentProblem:
         t = self.domain.t
         T0 = fwhm / (2.0 * np.log(1.0 + np.sqrt(2.0)))  # sech FWHM -> T0 (sech.py:111-113)
         tau = t[None, :] / T0[:, None]
-        A = np.sqrt(P0)[:, None] / np.cosh(tau) * np.exp(-0.5j * chirp[:, None] * tau ** 2)
+        e = np.exp(-np.abs(tau))  # sech = 2 e^-|tau| / (1 + e^-2|tau|), no overflow far out
+        A = np.sqrt(P0)[:, None] * (2.0 * e / (1.0 + e * e)) * np.exp(-0.5j * chirp[:, None] * tau ** 2)
         # one photon per mode, phases from Philox(key = seed * 2**20 + i)
         nu = fftshift(self.domain.nu)
         amp = np.sqrt(PLANCK_W_PS2 * nu / (self.n * self.domain.dt))
