@@ -212,6 +212,12 @@ int gnlse_memcpy_dtoh(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes)
 int gnlse_memcpy_dtod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
 int gnlse_propagate_adaptive_device(gnlse_ctx* ctx, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results);
+/* Engine-only (no pyofss counterpart): gather/scatter of `count` device rows
+ * of row_bytes (a multiple of 16) in one launch: row src_rows[r] of d_src to
+ * row dst_rows[r] of d_dst (host index arrays); the surrogate's judge moves
+ * flagged fields to and from the solver with it. */
+int gnlse_copy_rows(gnlse_ctx* ctx, int64_t count, int64_t row_bytes, const int64_t* src_rows, const int64_t* dst_rows,
+                    const void* d_src, void* d_dst);
 
 /* No counterpart in the reference. For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -326,6 +332,10 @@ int gnlse_sno_decode(gnlse_sno* s, gnlse_ctx* c, int64_t batch, const float* d_y
 /* Per-field relative change of energy between inputs d_x and predictions d_y
  * (the judge's invariant gate for a lossless segment), to host `defect`. */
 int gnlse_sno_energy_defect(gnlse_sno* s, int64_t batch, const float* d_x, const float* d_y, double* defect);
+/* Two networks' predictions d_y1, d_y2 (B x n x 2 FP32) of the same fields:
+ * d_ym = their mean and dist[b] (host) = |y1_b - y2_b| / |ym_b|, the
+ * disagreement the judge gates on. */
+int gnlse_sno_combine(gnlse_sno* s, int64_t batch, const float* d_y1, const float* d_y2, float* d_ym, double* dist);
 /* C (FP32) = op(A) op(B) on bf16 operands (device pointers), batched with
  * element strides sA, sB, sC and split over K; at/bt select stored-transposed
  * operands (not both). K / ksplit must be a multiple of 32. */

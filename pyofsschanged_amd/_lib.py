@@ -109,6 +109,7 @@ SIGNATURES = {
     "gnlse_memcpy_dtoh": [_P, _P, _P, _I64],
     "gnlse_memcpy_dtod": [_P, _P, _P, _I64],
     "gnlse_propagate_adaptive_device": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
+    "gnlse_copy_rows": [_P, _I64, _I64, _P, _P, _P, _P],
     "gnlse_propagate_adaptive_device_noalloc": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
     "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
@@ -134,6 +135,7 @@ SIGNATURES = {
     "gnlse_sno_encode": [_P, _P, _I64, _P, _D, _P, _INT, _INT, _P],
     "gnlse_sno_decode": [_P, _P, _I64, _P, _D, _P, _INT, _P],
     "gnlse_sno_energy_defect": [_P, _I64, _P, _P, _P],
+    "gnlse_sno_combine": [_P, _I64, _P, _P, _P, _P],
     "gnlse_check_reductions": [_INT, _INT, _INT, _I64, _P, _P, _P],
     "gnlse_gemm_bf16": [_INT, _INT, _INT, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _I64, _INT],
 }
@@ -430,6 +432,13 @@ class Context:
         self._pump_out(batch)
         return [{"z": r.z, "h": r.h, "n_attempts": r.n_attempts, "n_accepted": r.n_accepted,
                  "n_traces": r.n_traces, "status": r.status} for r in res]
+
+    def copy_rows(self, row_bytes, src_rows, dst_rows, d_src, d_dst):
+        """Row src_rows[r] of d_src to row dst_rows[r] of d_dst, one launch."""
+        s = np.ascontiguousarray(src_rows, dtype=np.int64)
+        d = np.ascontiguousarray(dst_rows, dtype=np.int64)
+        if s.size:
+            self._call("gnlse_copy_rows", s.size, int(row_bytes), _ptr(s), _ptr(d), d_src, d_dst)
 
     def propagate_adaptive_device_noalloc(self, method, batch, params, d_in, d_out, d_res):
         """Enqueue the no-allocation propagation; d_res receives batch

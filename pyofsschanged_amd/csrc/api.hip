@@ -189,7 +189,7 @@ int gnlse_ctx_destroy(gnlse_ctx* c) {
                     &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
                     &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr,
                     &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk, &c->yb_rs, &c->yb_ip, &c->yb_as, &c->yb_es,
-                    &c->yb_pp, &c->lin2, &c->efix2};
+                    &c->yb_pp, &c->lin2, &c->efix2, &c->rows};
   for (DevBuf* b : bufs) release(*b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -629,6 +629,35 @@ int gnlse_memcpy_dtod(gnlse_ctx* c, void* dst, const void* src, int64_t bytes) {
   if (!c || !dst || !src || bytes < 0) return fail(GNLSE_ERR_ARG, "invalid copy arguments");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GNLSE_OK;
+}
+
+// Row r of the copy: n2 double2 from src + srow[r] n2 to dst + drow[r] n2.
+__global__ void __launch_bounds__(256) k_copy_rows(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                  const int64_t* __restrict__ rows, long long count, long long n2) {
+  const long long r = blockIdx.x;
+  if (r >= count) return;
+  const double2* s = src + rows[r] * n2;
+  double2* d = dst + rows[count + r] * n2;
+  for (long long i = threadIdx.x; i < n2; i += 256) d[i] = s[i];
+}
+
+int gnlse_copy_rows(gnlse_ctx* c, int64_t count, int64_t row_bytes, const int64_t* src_rows, const int64_t* dst_rows,
+                    const void* d_src, void* d_dst) {
+  if (!c || count < 1 || count > 0x7fffffff || row_bytes < 16 || row_bytes % 16 || !src_rows || !dst_rows || !d_src ||
+      !d_dst)
+    return fail(GNLSE_ERR_ARG, "invalid row copy arguments");
+  for (int64_t r = 0; r < count; ++r)
+    if (src_rows[r] < 0 || dst_rows[r] < 0) return fail(GNLSE_ERR_ARG, "negative row index");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<int64_t> rows(2 * count);
+  std::memcpy(rows.data(), src_rows, count * sizeof(int64_t));
+  std::memcpy(rows.data() + count, dst_rows, count * sizeof(int64_t));
+  RET_IF(upload(c, c->rows, rows.data(), rows.size() * sizeof(int64_t)));
+  hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)count), dim3(256), 0, c->stream, (const double2*)d_src, (double2*)d_dst,
+                     (const int64_t*)c->rows.p, (long long)count, (long long)(row_bytes / 16));
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return GNLSE_OK;
 }

@@ -58,6 +58,7 @@ struct gnlse_ctx {
   DevBuf yb_pp;                        // pump power per instance
   int64_t yb_count = 0;                // instances with a pump power set
   DevBuf lin2, efix2;                  // WDM channel 1 linear factor and its fixed-step exponential
+  DevBuf rows;                         // row index table of gnlse_copy_rows
   void* scratch = nullptr;             // this device's no-allocation workspace (device address)
 };
 

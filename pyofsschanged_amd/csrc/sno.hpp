@@ -584,4 +584,34 @@ __global__ void __launch_bounds__(256) k_sno_energy_defect(const float* __restri
   if (threadIdx.x == 0) out[b] = s_in[0] > 0.0 ? fabs(s_out[0] - s_in[0]) / s_in[0] : 0.0;
 }
 
+// Two networks' predictions y1, y2 (B x n x 2 FP32) of the same fields:
+// ym = (y1 + y2) / 2 and the per-field disagreement |y1 - y2| / |ym| (the
+// judge's error estimate; it is calibrated against the solver on held-out
+// fields).
+__global__ void __launch_bounds__(256) k_sno_combine(const float* __restrict__ y1, const float* __restrict__ y2,
+                                                    float* __restrict__ ym, double* __restrict__ dist, long long N) {
+  __shared__ double s_d[256], s_m[256];
+  const long long b = blockIdx.x;
+  double dd = 0.0, mm = 0.0;
+  for (long long n = threadIdx.x; n < 2 * N; n += 256) {
+    const long long p = b * 2 * N + n;
+    const double a = y1[p], c = y2[p];
+    const double m = 0.5 * (a + c);
+    ym[p] = (float)m;
+    dd += (a - c) * (a - c);
+    mm += m * m;
+  }
+  s_d[threadIdx.x] = dd;
+  s_m[threadIdx.x] = mm;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s_d[threadIdx.x] += s_d[threadIdx.x + o];
+      s_m[threadIdx.x] += s_m[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dist[b] = s_m[0] > 0.0 ? sqrt(s_d[0] / s_m[0]) : 0.0;
+}
+
 }  // namespace gn

@@ -501,6 +501,18 @@ int gnlse_sno_energy_defect(gnlse_sno* s, int64_t batch, const float* d_x, const
   return GNLSE_OK;
 }
 
+int gnlse_sno_combine(gnlse_sno* s, int64_t batch, const float* d_y1, const float* d_y2, float* d_ym, double* dist) {
+  if (!s || !d_y1 || !d_y2 || !d_ym || !dist || batch < 1 || batch > s->maxB)
+    return fail(GNLSE_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(s->device));
+  hipLaunchKernelGGL(k_sno_combine, dim3((unsigned)batch), dim3(256), 0, s->stream, d_y1, d_y2, d_ym, (double*)s->rel.p,
+                     s->N);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dist, s->rel.p, batch * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return GNLSE_OK;
+}
+
 int gnlse_gemm_bf16(int device, int at, int bt, int64_t M, int64_t N, int64_t K, const void* d_A, int64_t lda,
                     const void* d_B, int64_t ldb, float* d_C, int64_t ldc, int64_t batch, int64_t sA, int64_t sB,
                     int64_t sC, int ksplit) {

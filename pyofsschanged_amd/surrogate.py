@@ -28,11 +28,16 @@ energy change of the prediction <= gate); predictions failing it are
 re-solved. A fixed fraction (every audit_every-th field) is re-solved as
 well and its relative spectral error |S_pred - S_ref| / |S_ref| (S =
 |A~|^2) measured; audited predictions above the tolerance are replaced by
-the solver's field and counted.
+the solver's field and counted. With two independently trained networks
+the output is their mean and their disagreement is a per-field error
+estimate; every field whose disagreement exceeds a gate calibrated on
+held-out solver runs is re-solved too (the certified judge: no held-out
+field above tolerance passes its gate).
 """
 
 import ctypes
 import math
+import sys
 import time
 
 import numpy as np
@@ -173,6 +178,13 @@ class SpectralOperator:
         self._call("gnlse_sno_adam", ctypes.c_float(lr), ctypes.c_float(beta1), ctypes.c_float(beta2),
                    ctypes.c_float(eps))
 
+    def combine(self, batch, d_y1, d_y2, d_ym):
+        """Mean of two networks' predictions into d_ym and their per-field
+        disagreement |y1 - y2| / |ym| (host array)."""
+        out = np.empty(batch, dtype=np.float64)
+        self._call("gnlse_sno_combine", int(batch), d_y1, d_y2, d_ym, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
     def energy_defect(self, batch, d_x, d_y):
         out = np.empty(batch, dtype=np.float64)
         self._call("gnlse_sno_energy_defect", int(batch), d_x, d_y, out.ctypes.data_as(ctypes.c_void_p))
@@ -296,7 +308,7 @@ class Dataset:
         A_out.free()
 
 
-def train(net, data, steps, batch=32, lr=1e-3, lr_end=1e-5, seed=0, log_every=0):
+def train(net, data, steps, batch=32, lr=1e-3, lr_end=1e-5, seed=0, log_every=0, log_file=None):
     """Adam with a cosine learning-rate schedule over `steps` minibatches of
     consecutive fields (the sampled dataset is already in random order).
     Returns the loss history."""
@@ -313,45 +325,84 @@ def train(net, data, steps, batch=32, lr=1e-3, lr_end=1e-5, seed=0, log_every=0)
         net.adam(rate)
         hist.append(loss)
         if log_every and (s % log_every == 0 or s == steps - 1):
-            print(f"  step {s}: loss {loss:.3e} lr {rate:.2e}", flush=True)
+            print(f"  step {s}: loss {loss:.3e} lr {rate:.2e}", file=log_file or sys.stdout, flush=True)
     return np.array(hist)
 
 
 class FibreSurrogate:
     """Network inference for whole batches of fields on the device, judged
-    by the FP64 solver (energy gate + audit) with re-solving as fallback."""
+    by the FP64 solver with re-solving as fallback.
 
-    def __init__(self, problem, net, scale, gate=0.005, audit_every=64, tol=0.01, chunk=None):
+    One network (fast judge): predictions failing the energy gate are
+    re-solved, and every audit_every-th field is re-solved and replaced when
+    its spectral error exceeds tol.
+
+    Two networks (`partner`, certified judge): the output is the mean of the
+    two predictions and their disagreement is the error estimate; every field
+    whose disagreement exceeds `dist_gate` is re-solved as well. calibrate()
+    sets dist_gate from a held-out set the solver has propagated: a margin
+    below the smallest disagreement of any held-out field whose error exceeds
+    tol, so that no held-out field above tol passes."""
+
+    def __init__(self, problem, net, scale, gate=0.005, audit_every=64, tol=0.01, chunk=None, partner=None):
         self.problem, self.net, self.scale = problem, net, scale
         self.gate, self.audit_every, self.tol = gate, audit_every, tol
+        self.partner = partner
+        self.dist_gate = None
         self.chunk = chunk or net.max_batch
         n = problem.n
         self.x = DeviceArray(problem.ctx, (self.chunk, n, net.in_channels), np.float32)
         self.y = DeviceArray(problem.ctx, (self.chunk, n, 2), np.float32)
+        if partner is not None:
+            self.y2 = DeviceArray(problem.ctx, (self.chunk, n, 2), np.float32)
+            self.ym = DeviceArray(problem.ctx, (self.chunk, n, 2), np.float32)
 
     def infer(self, d_in, d_out, count):
-        """Network prediction of `count` fields (device rows of complex128)
-        into d_out; returns the per-field energy defect."""
+        """Prediction of `count` fields (device rows of complex128) into
+        d_out; returns the per-field energy defect and, with a partner
+        network, the disagreement (else None)."""
         p, net, n = self.problem, self.net, self.problem.n
         defect = np.empty(count)
+        dist = np.empty(count) if self.partner is not None else None
         row = n * 16
         for s0 in range(0, count, self.chunk):
             m = min(self.chunk, count - s0)
             p.encode(net, m, _off(d_in, row * s0), self.scale, self.x.ptr, channels=net.in_channels)
             net.forward(m, self.x.ptr, self.y.ptr)
-            defect[s0:s0 + m] = net.energy_defect(m, self.x.ptr, self.y.ptr)
-            p.decode(net, m, self.y.ptr, self.scale, _off(d_out, row * s0))
-        return defect
+            y = self.y
+            if self.partner is not None:
+                self.partner.forward(m, self.x.ptr, self.y2.ptr)
+                dist[s0:s0 + m] = net.combine(m, self.y.ptr, self.y2.ptr, self.ym.ptr)
+                y = self.ym
+            defect[s0:s0 + m] = net.energy_defect(m, self.x.ptr, y.ptr)
+            p.decode(net, m, y.ptr, self.scale, _off(d_out, row * s0))
+        return defect, dist
+
+    def calibrate(self, d_in, d_out, count, reference, margin=0.8):
+        """Set dist_gate from `count` held-out fields (device inputs d_in,
+        scratch d_out) whose solver outputs `reference` (host) are known."""
+        _, dist = self.infer(d_in, d_out, count)
+        pred = np.stack([self._row(d_out, i) for i in range(count)])
+        err = spectral_error(pred, reference)
+        above = err > self.tol
+        self.dist_gate = margin * float(dist[above].min()) if above.any() else float(dist.max())
+        return dict(fields=count, above_tol=int(above.sum()), dist_gate=self.dist_gate, margin=margin,
+                    flagged=float(np.mean(dist > self.dist_gate)))
 
     def propagate(self, d_in, d_out, count, reference=None):
         """Surrogate propagation with judging. `reference` (host array of
         solver outputs), when given, is used only to report the true error
         of every field, not by the pipeline. Returns a report dict."""
         p, ctx, n = self.problem, self.problem.ctx, self.problem.n
+        if self.partner is not None and self.dist_gate is None:
+            raise RuntimeError("the certified judge needs calibrate() first")
         t0 = time.perf_counter()
-        defect = self.infer(d_in, d_out, count)
+        defect, dist = self.infer(d_in, d_out, count)
         t_net = time.perf_counter() - t0
-        flagged = np.nonzero(defect > self.gate)[0]
+        flag = defect > self.gate
+        if dist is not None:
+            flag |= dist > self.dist_gate
+        flagged = np.nonzero(flag)[0]
         audit = np.arange(0, count, self.audit_every)
         resolve = np.union1d(flagged, audit)
         replaced = np.zeros(count, dtype=bool)
@@ -360,21 +411,20 @@ class FibreSurrogate:
             row = n * 16
             buf_in = DeviceArray(ctx, (resolve.size, n), np.complex128)
             buf_out = DeviceArray(ctx, (resolve.size, n), np.complex128)
-            for j, i in enumerate(resolve):
-                ctx.dtod(buf_in.at(j), _off(d_in, row * int(i)), row)
+            ctx.copy_rows(row, resolve, np.arange(resolve.size), d_in, buf_in.ptr)
             p.solve(buf_in.ptr, buf_out.ptr, resolve.size)
-            is_audit = np.isin(resolve, audit)
+            is_audit = np.isin(resolve, audit) & ~np.isin(resolve, flagged)
             # audited predictions are compared with the solver on the host
             aud = np.nonzero(is_audit)[0]
-            bad = ~is_audit  # gate failures are always replaced
+            bad = ~is_audit  # flagged fields are always replaced
             if aud.size:
                 preds = np.stack([self._row(d_out, resolve[j]) for j in aud])
                 ref = np.stack([buf_out.download(int(j), 1)[0] for j in aud])
                 audit_err = spectral_error(preds, ref)
                 bad[aud] = audit_err > self.tol
-            for j in np.nonzero(bad)[0]:
-                ctx.dtod(_off(d_out, row * int(resolve[j])), buf_out.at(int(j)), row)
-                replaced[resolve[j]] = True
+            jb = np.nonzero(bad)[0]
+            ctx.copy_rows(row, jb, resolve[jb], buf_out.ptr, d_out)
+            replaced[resolve[jb]] = True
             buf_in.free()
             buf_out.free()
         t_total = time.perf_counter() - t0

@@ -13,6 +13,10 @@ references.
 * Training on solver-generated data lowers the loss, and the trained network
   is more accurate than the method without it (linear propagation only,
   which is what the untrained residual network computes).
+* The certified judge: the two-network combine kernel against numpy (mean
+  to FP32 rounding, disagreement 1e-6 relative), and on the held-out fields
+  it was calibrated on, no delivered field exceeds the tolerance (every
+  field above it is re-solved by construction of the gate).
 """
 
 import numpy as np
@@ -165,3 +169,39 @@ def test_training_on_solver_data(env):
           f"train loss {hist[0]:.3e} -> {hist[-10:].mean():.3e}", flush=True)
     assert hist[-10:].mean() < 0.2 * hist[:10].mean()
     assert np.median(rel1) < 0.5 * np.median(rel0)
+
+
+def test_certified_judge(env):
+    _, S, ctx = env
+    prob = S.SegmentProblem()
+    nets = []
+    for k in range(2):
+        net = S.SpectralOperator(n=prob.n, max_batch=256)
+        net.init_params(seed=3 + k)
+        nets.append(net)
+    train = S.Dataset(prob, nets[0], 1024, seed=11)
+    for k, net in enumerate(nets):
+        S.train(net, train, steps=300, batch=32, lr=2e-3, seed=k)
+    # combine kernel against numpy on the two networks' raw outputs
+    held = S.Dataset(prob, nets[0], 256, seed=12, scale=train.scale, keep_fields=True)
+    y1 = S.DeviceArray(ctx, (256, prob.n, 2), np.float32)
+    y2 = S.DeviceArray(ctx, (256, prob.n, 2), np.float32)
+    ym = S.DeviceArray(ctx, (256, prob.n, 2), np.float32)
+    nets[0].forward(256, held.x.ptr, y1.ptr)
+    nets[1].forward(256, held.x.ptr, y2.ptr)
+    dist = nets[0].combine(256, y1.ptr, y2.ptr, ym.ptr)
+    a, b, m = y1.download().astype(np.float64), y2.download().astype(np.float64), ym.download().astype(np.float64)
+    assert np.allclose(m, 0.5 * (a + b), rtol=1e-6, atol=1e-7 * np.abs(m).max())
+    ref_d = np.linalg.norm((a - b).reshape(256, -1), axis=1) / np.linalg.norm(m.reshape(256, -1), axis=1)
+    assert np.allclose(dist, ref_d, rtol=1e-6)
+    # calibration, then judging the same held-out fields: nothing above tol passes
+    sur = S.FibreSurrogate(prob, nets[0], train.scale, partner=nets[1], chunk=256)
+    d_in = S.DeviceArray(ctx, held.fields_in.shape, np.complex128)
+    d_out = S.DeviceArray(ctx, held.fields_in.shape, np.complex128)
+    d_in.upload(held.fields_in)
+    cal = sur.calibrate(d_in.ptr, d_out.ptr, 256, held.fields_out)
+    rep = sur.propagate(d_in.ptr, d_out.ptr, 256, reference=held.fields_out)
+    print(f"certified judge: calibration {cal}, within tol {rep['within_tol']:.3f}, resolved {rep['resolved']}",
+          flush=True)
+    assert rep["within_tol"] == 1.0
+    assert rep["err_max"] <= sur.tol
