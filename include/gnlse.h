@@ -204,7 +204,8 @@ int gnlse_propagate_adaptive(gnlse_ctx* ctx, int method, int64_t batch, const gn
                              double* log_delta, uint8_t* log_accept, gnlse_instance_result* results);
 
 /* Device-memory helpers and device-resident adaptive propagation (inputs
- * already in HBM; no traces, no log). results is a host array. */
+ * already in HBM; no traces, no log). results is a host array; d_A_in and
+ * d_A_out must not overlap (GNLSE_ERR_ARG otherwise). */
 int gnlse_malloc(gnlse_ctx* ctx, int64_t bytes, void** dptr);
 int gnlse_free(gnlse_ctx* ctx, void* dptr);
 int gnlse_memcpy_htod(gnlse_ctx* ctx, void* dst, const void* src, int64_t bytes);
@@ -228,8 +229,9 @@ int gnlse_copy_rows(gnlse_ctx* ctx, int64_t count, int64_t row_bytes, const int6
  * d_results (batch entries). The call allocates nothing, enqueues one launch
  * on the context's stream and returns without waiting (gnlse_memcpy_dtoh on
  * the same context waits for it). Rejected with GNLSE_ERR_ARG when batch x
- * 176 n bytes (twice that for WDM fields) exceeds the array: 45 instances at
- * n = 8192. Every call on one GPU uses that GPU's one array: calls that
+ * 176 n bytes (192 n on the 32768- and 65536-sample grids; twice that for
+ * WDM fields) exceeds the array (45 instances at n = 8192, 10 at n = 32768)
+ * and when d_A_in and d_A_out overlap. Every call on one GPU uses that GPU's one array: calls that
  * overlap on a GPU (different contexts or threads) race. The operator
  * set-up calls (gnlse_set_grid, _linear, _nonlinear, _gain, _pump) still
  * allocate and must come first. */

@@ -662,6 +662,17 @@ int gnlse_copy_rows(gnlse_ctx* c, int64_t count, int64_t row_bytes, const int64_
   return GNLSE_OK;
 }
 
+// The propagation kernels take input and output as __restrict__ pointers
+// (each workgroup reads its instance once and writes it once, after all of
+// its steps): the two device ranges of a batch must not overlap.
+static int check_disjoint(const gnlse_ctx* c, int64_t batch, const double* in, const double* out) {
+  const uintptr_t bytes = (uintptr_t)batch * fields_per_instance(c) * c->n * sizeof(double2);
+  const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+  if (a < b + bytes && b < a + bytes)
+    return fail(GNLSE_ERR_ARG, "d_A_in and d_A_out overlap; give the output its own device buffer");
+  return GNLSE_OK;
+}
+
 int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
                                     const double* d_A_in, double* d_A_out, gnlse_instance_result* results) {
   RET_IF(check_ready(c, true, true));
@@ -669,6 +680,7 @@ int gnlse_propagate_adaptive_device(gnlse_ctx* c, int method, int64_t batch, con
   RET_IF(adaptive_params(method, p, prm));
   if (batch <= 0 || batch > 0x7fffffff || !d_A_in || !d_A_out || !results)
     return fail(GNLSE_ERR_ARG, "invalid arguments");
+  RET_IF(check_disjoint(c, batch, d_A_in, d_A_out));
   HIPCHK(hipSetDevice(c->device));
   RET_IF(ensure(c->res, (size_t)batch * sizeof(gnlse_instance_result)));
   prm.res = (gnlse_instance_result*)c->res.p;
@@ -686,6 +698,7 @@ int gnlse_propagate_adaptive_device_noalloc(gnlse_ctx* c, int method, int64_t ba
   RET_IF(adaptive_params(method, p, prm));
   if (batch <= 0 || batch > 0x7fffffff || !d_A_in || !d_A_out || !d_results)
     return fail(GNLSE_ERR_ARG, "invalid arguments");
+  RET_IF(check_disjoint(c, batch, d_A_in, d_A_out));
   RET_IF(check_pump(c, batch));
   const size_t per = (size_t)fields_per_instance(c) * c->tab->nslot * c->n * sizeof(double2);
   if ((size_t)batch * per > (size_t)GNLSE_NOALLOC_SCRATCH_BYTES)

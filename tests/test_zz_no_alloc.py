@@ -96,5 +96,22 @@ def test_noalloc_rejects_batch_beyond_the_array(setup):
     too_many = (64 * 1024 * 1024) // per + 1
     with pytest.raises(lib.GNLSEError, match="no-allocation entry point holds"):
         # rejected on the host before any launch: the pointers are never read
-        ctx.propagate_adaptive_device_noalloc(0, too_many, p, ctypes.c_void_p(16), ctypes.c_void_p(16),
+        ctx.propagate_adaptive_device_noalloc(0, too_many, p, ctypes.c_void_p(16), ctypes.c_void_p(16 + (1 << 44)),
                                               ctypes.c_void_p(16))
+
+
+@pytest.mark.parametrize("offset", [0, N * 16 - 16, -(N * 16 - 16)], ids=["same", "tail", "head"])
+def test_overlapping_input_and_output_rejected(setup, offset):
+    # the kernels take input and output as restrict pointers; both device
+    # entry points refuse ranges that share a byte, before any launch
+    lib, dom, A, ctx, p = setup
+    base = ctx.malloc(4 * N * 16)
+    try:
+        d_in = ctypes.c_void_p(base.value + N * 16)
+        d_out = ctypes.c_void_p(base.value + N * 16 + offset)
+        with pytest.raises(lib.GNLSEError, match="overlap"):
+            ctx.propagate_adaptive_device(0, 1, p, d_in, d_out)
+        with pytest.raises(lib.GNLSEError, match="overlap"):
+            ctx.propagate_adaptive_device_noalloc(0, 1, p, d_in, d_out, ctypes.c_void_p(base.value + 3 * N * 16))
+    finally:
+        ctx.free(base)
