@@ -18,8 +18,11 @@ against 8 TB/s and the fft_counter work as bytes), a CPU baseline (the NumPy
 oracle on one host core running the whole propagation of realisation 0,
 unscaled; beside it, "all_cores": a bounded sample in one single-threaded
 process per core of the GPU's 16-core share, scaled) and "error", the
-relative L2 field error of realisation 0 against that oracle run. Further
-objects report the other workloads of the engine:
+relative L2 field error of realisation 0 against that oracle run. Before
+the timed launches the autotuner picks the headline kernel's schedule
+(threads per instance, transform pass plans, stage block) from trial runs on
+the same inputs; "autotuner" records the choice and every candidate's trial
+time. Further objects report the other workloads of the engine:
 "ensemble" (512 noise-seeded soliton realisations on 4096/8192/16384-sample
 grids in one graph-replayed launch with |g12| on the device), "amplifier"
 (64 chirped pulses through the 5 m saturable-gain amplifier, 16384 samples),
@@ -32,7 +35,7 @@ gated and audited fields).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
                     [--no-ensemble] [--no-amplifier] [--no-yb] [--no-surrogate]
-                    [--surrogate-steps S]
+                    [--surrogate-steps S] [--no-tune | --schedule I]
 """
 
 import argparse
@@ -67,6 +70,8 @@ def parse():
     ap.add_argument("--no-amplifier", action="store_true", help="skip the amplifier-batch measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training and measurement")
     ap.add_argument("--no-yb", action="store_true", help="skip the Yb attractor measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default schedules (no autotuner)")
+    ap.add_argument("--schedule", type=int, default=None, help="headline schedule index (skips its tuning)")
     ap.add_argument("--surrogate-steps", type=int, default=60000, help="training steps of the surrogate")
     ap.add_argument("--surrogate-train", type=int, default=20000, help="solver runs in the training set")
     ap.add_argument("--surrogate-length", type=float, default=1e-3, help="surrogate fibre segment (km)")
@@ -206,7 +211,21 @@ def ensemble_measurement():
             "graph": True}
 
 
-def amplifier_measurement():
+def tune(ctx, fields, params, trial_attempts=100):
+    """The autotuner on a context: every compiled schedule of its grid runs
+    trial_attempts replayed steps of h0 on these fields (three runs, the first
+    untimed); the fastest stays selected. Returns the record for the JSON line."""
+    from pyofsschanged_amd import _lib
+    idx, trial = ctx.tune_adaptive(fields, params, trial_attempts=trial_attempts, repeats=3)
+    sc = _lib.schedules(ctx.n)
+    print(f"autotuner ({ctx.n} samples): {sc[idx]['desc']}", file=sys.stderr, flush=True)
+    return {"chosen": sc[idx]["desc"], "index": idx,
+            "trial_ms": {s["desc"]: round(m, 3) for s, m in zip(sc, trial)},
+            "trial": f"{trial_attempts} replayed attempts of h0 per instance, {fields.shape[0]} instances, mean of "
+                     f"two timed runs per schedule"}
+
+
+def amplifier_measurement(do_tune=True):
     """64 chirped Gaussian pulses through the 5 m saturable-gain amplifier
     (16384 samples, Raman, Lorentzian gain) in one adaptive launch."""
     from pyofsschanged_amd import workloads
@@ -218,6 +237,7 @@ def amplifier_measurement():
     d_in = ctx.malloc(fields.nbytes)
     d_out = ctx.malloc(fields.nbytes)
     ctx.htod(d_in, fields)
+    sched = tune(ctx, fields, params, trial_attempts=40) if do_tune else None
     ctx.propagate_adaptive_device(METHOD_RK4IP, B, params, d_in, d_out)  # warm-up
     t0 = time.perf_counter()
     res = ctx.propagate_adaptive_device(METHOD_RK4IP, B, params, d_in, d_out)
@@ -235,7 +255,7 @@ def amplifier_measurement():
                         "[-2, 2], 5 m, beta2 0.023 ps^2/m, gamma 5e-3 /W/m, Raman, 15 dB gain, E_sat 5 nJ, "
                         "Lorentzian 40 nm gain, adaptive RK4IP",
             "batch": B, "launch_ms": ms, "propagations_per_s": B / wall, "mean_attempts": att,
-            "algorithmic_GBps": achieved, "hbm_frac": achieved / HBM_PEAK_GBS}
+            "algorithmic_GBps": achieved, "hbm_frac": achieved / HBM_PEAK_GBS, "schedule": sched}
 
 
 def yb_attractor_measurement(batch):
@@ -362,7 +382,7 @@ def main():
               f"with no exchange until ensemble statistics)", file=sys.stderr)
         sys.exit(2)
 
-    from pyofsschanged_amd import workloads
+    from pyofsschanged_amd import _lib, workloads
     from pyofsschanged_amd._lib import METHOD_RK4IP
 
     domain, pulse, fibre = workloads.supercontinuum(length=args.length)
@@ -377,6 +397,11 @@ def main():
     d_in = ctx.malloc(nbytes)
     d_out = ctx.malloc(nbytes)
     ctx.htod(d_in, inputs)
+    sched = None
+    if args.schedule is not None:
+        ctx.set_schedule(args.schedule)
+    elif not args.no_tune:
+        sched = tune(ctx, inputs, params)
 
     for _ in range(args.warmup):
         ctx.propagate_adaptive_device(METHOD_RK4IP, B, params, d_in, d_out)
@@ -442,7 +467,9 @@ def main():
             "batch": B,
             "length_km": args.length,
             "mean_attempts_per_propagation": float(np.mean(attempts)) / B,
+            "schedule": _lib.schedules(N)[ctx.get_schedule()]["desc"],
         },
+        "autotuner": sched,
         "roofline": {
             "bound": "fp64",
             "achieved": achieved_tflops,
@@ -466,7 +493,7 @@ def main():
     if not args.no_ensemble:
         line["ensemble"] = ensemble_measurement()
     if not args.no_amplifier:
-        line["amplifier"] = amplifier_measurement()
+        line["amplifier"] = amplifier_measurement(not args.no_tune)
     if not args.no_yb:
         line["yb_attractor"] = yb_attractor_measurement(cu_count())
     if not args.no_surrogate:

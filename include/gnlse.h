@@ -267,6 +267,34 @@ int gnlse_trace_powers(gnlse_ctx* ctx, int64_t count, const double* A, int64_t n
  * the context's stream around the launch. */
 int gnlse_last_kernel_ms(gnlse_ctx* ctx, double* ms);
 
+/* Schedules and their autotuner (engine-only; pyofss has one NumPy path).
+ * The adaptive RK4IP launches with the full Raman + self-steepening operator
+ * (no gain, or the saturable gain; Stepper.adaptive_stepper, stepper.py:
+ * 247-376, over Solver.rk4ip, solver.py:339-354) are compiled in several
+ * schedules per grid: threads per instance (waves per instance x 64), the
+ * pass plan of the N-point transform and of the N/2-point transform of the
+ * Raman convolution, and the block of the pointwise stage passes. Schedule 0
+ * is the grid's default. All schedules compute the same arithmetic up to the
+ * order of the transforms' roundings. */
+/* Number of schedules of an n-sample grid (1 where no alternative is
+ * compiled); no GPU needed. */
+int gnlse_schedule_count(int64_t n, int* count);
+/* threads per instance, CFG code and a description ("T512 N:32L.16.16
+ * R:8.8.8.8 tile4"; radices of the passes, L = lane-pair pass) of schedule
+ * `index` of an n-sample grid; no GPU needed. */
+int gnlse_schedule_info(int64_t n, int index, int* threads, int* cfg, char* desc, int desc_len);
+/* Select the schedule of the context's launches (0 = default) / read it. */
+int gnlse_set_schedule(gnlse_ctx* ctx, int index);
+int gnlse_get_schedule(gnlse_ctx* ctx, int* index);
+/* Autotuner: times every schedule of the context's grid on trial_attempts
+ * replayed steps of p->h0 (all accepted) of `batch` instances of the host
+ * inputs A_in, `repeats` runs each (the first one untimed), selects the
+ * fastest for the context and returns its index in *chosen and, when
+ * ms is not NULL, each schedule's time (ms[count]). GNLSE_ERR_UNSUPPORTED
+ * when the context's operators are not a scheduled configuration. */
+int gnlse_tune_adaptive(gnlse_ctx* ctx, int64_t batch, const gnlse_adaptive_params* p, const double* A_in,
+                        int64_t trial_attempts, int repeats, int* chosen, double* ms);
+
 /* Noise ensembles on mixed grids (engine-only: pyofss runs one System per
  * realisation; this replaces that loop for the stepping path). Groups are up
  * to four sets of realisations, each on the grid (4096, 8192 or 16384) and

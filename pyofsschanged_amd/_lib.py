@@ -112,6 +112,13 @@ SIGNATURES = {
     "gnlse_copy_rows": [_P, _I64, _I64, _P, _P, _P, _P],
     "gnlse_propagate_adaptive_device_noalloc": [_P, _INT, _I64, ctypes.POINTER(AdaptiveParams), _P, _P, _P],
     "gnlse_last_kernel_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gnlse_schedule_count": [_I64, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_schedule_info": [_I64, _INT, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                            _INT],
+    "gnlse_set_schedule": [_P, _INT],
+    "gnlse_get_schedule": [_P, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_tune_adaptive": [_P, _I64, ctypes.POINTER(AdaptiveParams), _P, _I64, _INT, ctypes.POINTER(ctypes.c_int),
+                            _P],
     "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
     "gnlse_trace_powers": [_P, _I64, _P, _I64, _INT, _P, _P],
     "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
@@ -179,6 +186,23 @@ def device_count():
 
 def grid_supported(n):
     return bool(load_library().gnlse_grid_supported(int(n)))
+
+
+def schedules(n):
+    """The compiled schedules of the n-sample grid's spectral-domain adaptive
+    RK4IP kernels: list of dicts (index, threads, cfg, desc); index 0 is the
+    grid's default. No GPU needed."""
+    lib = load_library()
+    c = ctypes.c_int()
+    _check(lib, lib.gnlse_schedule_count(int(n), ctypes.byref(c)), "gnlse_schedule_count")
+    out = []
+    for i in range(c.value):
+        t, cfg = ctypes.c_int(), ctypes.c_int()
+        buf = ctypes.create_string_buffer(128)
+        _check(lib, lib.gnlse_schedule_info(int(n), i, ctypes.byref(t), ctypes.byref(cfg), buf, 128),
+               "gnlse_schedule_info")
+        out.append({"index": i, "threads": t.value, "cfg": cfg.value, "desc": buf.value.decode()})
+    return out
 
 
 def _c128(a, n=None):
@@ -445,6 +469,28 @@ class Context:
         InstanceResult records in device memory (read them with dtoh)."""
         self._call("gnlse_propagate_adaptive_device_noalloc", int(method), int(batch), ctypes.byref(params), d_in,
                    d_out, d_res)
+
+    def set_schedule(self, index):
+        """Select the schedule of the spectral-domain RK4IP launches (0 is
+        the grid's default; see schedules())."""
+        self._call("gnlse_set_schedule", int(index))
+
+    def get_schedule(self):
+        v = ctypes.c_int()
+        self._call("gnlse_get_schedule", ctypes.byref(v))
+        return v.value
+
+    def tune_adaptive(self, A, params, trial_attempts=40, repeats=3):
+        """Autotuner: time every schedule of this grid on trial_attempts
+        replayed steps of params.h0 over the rows of A (repeats runs each, the
+        first untimed), keep the fastest for this context. Returns (index,
+        list of ms per schedule)."""
+        a, _ = self._batched(A)
+        c = ctypes.c_int()
+        ms = np.zeros(len(schedules(self.n)), dtype=np.float64)
+        self._call("gnlse_tune_adaptive", a.shape[0], ctypes.byref(params), _ptr(a), int(trial_attempts), int(repeats),
+                   ctypes.byref(c), _ptr(ms))
+        return c.value, ms.tolist()
 
     def last_kernel_ms(self):
         v = ctypes.c_double()

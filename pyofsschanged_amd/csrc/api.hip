@@ -28,6 +28,49 @@ const LaunchTable* find_table(long long n) {
     if (t->n == n) return t;
   return nullptr;
 }
+
+// Alternative schedules of the spectral-domain adaptive RK4IP kernels
+// (sched_<N>.hip); schedule 0 of a grid is its launch table's own.
+#define GN_SCHEDULES(X) \
+  X(4096, 1) X(4096, 2) X(4096, 3) X(8192, 1) X(8192, 2) X(8192, 3) X(8192, 4) X(16384, 1) X(16384, 2)
+#define GN_DECLARE_SCHEDULE(NN, ID) const Schedule* schedule_##NN##_##ID();
+GN_SCHEDULES(GN_DECLARE_SCHEDULE)
+#undef GN_DECLARE_SCHEDULE
+
+struct ScheduleEntry {
+  long long n;
+  const Schedule* (*get)();
+};
+static std::vector<const Schedule*> grid_schedules(long long n) {
+#define GN_SCHEDULE_ENTRY(NN, ID) {NN, &schedule_##NN##_##ID},
+  static const ScheduleEntry all[] = {GN_SCHEDULES(GN_SCHEDULE_ENTRY)};
+#undef GN_SCHEDULE_ENTRY
+  std::vector<const Schedule*> v;
+  for (const ScheduleEntry& e : all)
+    if (e.n == n) v.push_back(e.get());
+  return v;
+}
+
+// "T512 N:32L.16.16 R:8.8.8.8 tile4": threads, the pass radices of the
+// N-point and N/2-point transforms (L: lane-pair pass) and the stage block.
+static std::string schedule_desc(long long n, int t, int cfg) {
+  const int renum = ((cfg & 8) || (cfg & 0x80)) ? 16 : 0;
+  auto plan = [&](long long len, int pl) {
+    const int p = (int)(len / t);
+    const Plan pn = make_plan((int)len, p, pl);
+    std::string s;
+    for (int i = 0; i < pn.npass; ++i) {
+      if (i) s += ".";
+      s += std::to_string(pn.radix[i]);
+      if (pn.radix[i] == 2 * p && (pl & 8)) s += "L";
+    }
+    return s;
+  };
+  const int tb = (cfg >> 8) & 3;
+  const int tile = tb == 1 ? 2 : tb == 2 ? 8 : 4;
+  return "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) + " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) +
+         " tile" + std::to_string(tile);
+}
 }  // namespace gn
 
 using namespace gn;
@@ -206,6 +249,7 @@ int gnlse_set_grid(gnlse_ctx* c, int64_t n) {
   if (c->n != n) {
     c->n = n;
     c->tab = t;
+    c->sched = 0;
     c->have_lin = c->have_nl = c->have_lin2 = false;
     c->efix_h = NAN;
     RET_IF(build_twiddles(c));
@@ -525,12 +569,23 @@ static int adaptive_params(int method, const gnlse_adaptive_params* p, AdaptiveP
   return GNLSE_OK;
 }
 
+// The adaptive launch through the context's schedule (the grid's table for
+// schedule 0 and for launches no schedule covers).
+static hipError_t adaptive_enqueue(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm,
+                                   const double2* in, double2* out, double2* ws) {
+  if (c->sched > 0 && schedule_applies(c->ops, method)) {
+    const std::vector<const Schedule*> v = grid_schedules(c->n);
+    if (c->sched <= (int)v.size()) return v[c->sched - 1]->adaptive(c->stream, c->ops, prm, batch, in, out, ws);
+  }
+  return c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, ws);
+}
+
 static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm, const double2* in,
                            double2* out) {
   RET_IF(check_pump(c, batch));
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
-  return timed_end(c, c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p));
+  return timed_end(c, adaptive_enqueue(c, method, batch, prm, in, out, (double2*)c->ws.p));
 }
 
 int gnlse_propagate_adaptive(gnlse_ctx* c, int method, int64_t batch, const gnlse_adaptive_params* p,
@@ -710,9 +765,96 @@ int gnlse_propagate_adaptive_device_noalloc(gnlse_ctx* c, int method, int64_t ba
   if (!c->scratch) return fail(GNLSE_ERR_STATE, "no workspace address for this device");
   HIPCHK(hipSetDevice(c->device));
   prm.res = d_results;
-  const hipError_t e = c->tab->adaptive(c->stream, c->ops, method, prm, batch, (const double2*)d_A_in,
-                                        (double2*)d_A_out, (double2*)c->scratch);
+  const hipError_t e = adaptive_enqueue(c, method, batch, prm, (const double2*)d_A_in, (double2*)d_A_out,
+                                        (double2*)c->scratch);
   if (e != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return GNLSE_OK;
+}
+
+int gnlse_schedule_count(int64_t n, int* count) {
+  if (!count) return fail(GNLSE_ERR_ARG, "null count");
+  if (!find_table(n)) return fail(GNLSE_ERR_UNSUPPORTED, "grid size " + std::to_string(n) + " has no kernel plan");
+  *count = 1 + (int)grid_schedules(n).size();
+  return GNLSE_OK;
+}
+
+int gnlse_schedule_info(int64_t n, int index, int* threads, int* cfg, char* desc, int desc_len) {
+  const LaunchTable* t = find_table(n);
+  if (!t) return fail(GNLSE_ERR_UNSUPPORTED, "grid size " + std::to_string(n) + " has no kernel plan");
+  const std::vector<const Schedule*> v = grid_schedules(n);
+  if (index < 0 || index > (int)v.size()) return fail(GNLSE_ERR_ARG, "schedule index out of range");
+  const int th = index == 0 ? t->t : v[index - 1]->t;
+  const int cf = index == 0 ? t->cfg : v[index - 1]->cfg;
+  if (threads) *threads = th;
+  if (cfg) *cfg = cf;
+  if (desc && desc_len > 0) {
+    const std::string d = t->blocks > 1 ? "T" + std::to_string(th) + " two-level" : schedule_desc(n, th, cf);
+    std::strncpy(desc, d.c_str(), (size_t)desc_len - 1);
+    desc[desc_len - 1] = 0;
+  }
+  return GNLSE_OK;
+}
+
+int gnlse_set_schedule(gnlse_ctx* c, int index) {
+  RET_IF(check_ready(c, false, false));
+  if (index < 0 || index > (int)grid_schedules(c->n).size()) return fail(GNLSE_ERR_ARG, "schedule index out of range");
+  c->sched = index;
+  return GNLSE_OK;
+}
+
+int gnlse_get_schedule(gnlse_ctx* c, int* index) {
+  if (!c || !index) return fail(GNLSE_ERR_ARG, "null argument");
+  *index = c->sched;
+  return GNLSE_OK;
+}
+
+int gnlse_tune_adaptive(gnlse_ctx* c, int64_t batch, const gnlse_adaptive_params* p, const double* A_in,
+                        int64_t trial_attempts, int repeats, int* chosen, double* ms) {
+  RET_IF(check_ready(c, true, true));
+  if (!chosen || trial_attempts < 1 || repeats < 2) return fail(GNLSE_ERR_ARG, "invalid tuning arguments");
+  if (!schedule_applies(c->ops, GNLSE_METHOD_RK4IP))
+    return fail(GNLSE_ERR_UNSUPPORTED, "no schedules for these operators (rk4ip with the full Raman + "
+                                       "self-steepening operator, no gain or the saturable gain)");
+  AdaptiveParams prm;
+  RET_IF(adaptive_params(GNLSE_METHOD_RK4IP, p, prm));
+  HIPCHK(hipSetDevice(c->device));
+  RET_IF(io_prepare(c, batch, A_in, c->io_in));
+  RET_IF(ensure(c->io_out, (size_t)batch * c->n * sizeof(double2)));
+  RET_IF(ensure(c->res, (size_t)batch * sizeof(gnlse_instance_result)));
+  // every schedule runs the same attempts: steps of h0, all accepted
+  const std::vector<double> rh((size_t)trial_attempts, p->h0);
+  const std::vector<unsigned char> ra((size_t)trial_attempts, 1);
+  RET_IF(upload(c, c->rp_h, rh.data(), rh.size() * sizeof(double)));
+  RET_IF(upload(c, c->rp_a, ra.data(), ra.size()));
+  prm.replay_h = (const double*)c->rp_h.p;
+  prm.replay_accept = (const unsigned char*)c->rp_a.p;
+  prm.replay_len = trial_attempts;
+  prm.res = (gnlse_instance_result*)c->res.p;
+  const int keep = c->sched;
+  const int count = 1 + (int)grid_schedules(c->n).size();
+  int best = 0;
+  double best_ms = 0.0;
+  for (int i = 0; i < count; ++i) {
+    c->sched = i;
+    double sum = 0.0;
+    for (int r = 0; r < repeats; ++r) {  // the first run loads the code object
+      const int rc = launch_adaptive(c, GNLSE_METHOD_RK4IP, batch, prm, (const double2*)c->io_in.p,
+                                     (double2*)c->io_out.p);
+      if (rc != GNLSE_OK) {
+        c->sched = keep;
+        return rc;
+      }
+      if (r > 0) sum += c->last_ms;
+    }
+    const double t = sum / (repeats - 1);
+    if (ms) ms[i] = t;
+    if (i == 0 || t < best_ms) {
+      best = i;
+      best_ms = t;
+    }
+  }
+  c->sched = best;
+  *chosen = best;
   return GNLSE_OK;
 }
 
@@ -907,7 +1049,7 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
   for (int a = 1; a < ngroups; ++a)
     for (int b = a; b > 0 && ctxs[order[b]]->n > ctxs[order[b - 1]]->n; --b) std::swap(order[b], order[b - 1]);
   size_t fields = 0, g12s = 0, ress = 0;
-  bool all_kerr = true;
+  bool all_kerr = true, all_sp = method == GNLSE_METHOD_RK4IP;
   for (int s = 0; s < ngroups; ++s) {
     const int i = order[s];
     e->slot[i] = s;
@@ -920,8 +1062,10 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
     g12s += (size_t)ctxs[i]->n;
     ress += (size_t)batch[i];
     all_kerr = all_kerr && ctxs[i]->ops.nl_kind == NL_KERR;
+    all_sp = all_sp && ctxs[i]->ops.nl_kind == NL_ALL && ctxs[i]->ops.gain == GAIN_NONE && !ctxs[i]->ops.wdm;
   }
-  e->kind = all_kerr ? NL_KERR : NL_GENERIC;
+  // the spectral-domain RK4IP when every group has the full operator without gain
+  e->kind = all_kerr ? NL_KERR : all_sp ? NL_ALL : NL_GENERIC;
   int rc = GNLSE_OK;
   auto alloc = [&](DevBuf& b, size_t bytes) {
     if (rc == GNLSE_OK) rc = ensure(b, bytes);

@@ -60,6 +60,7 @@ struct gnlse_ctx {
   DevBuf lin2, efix2;                  // WDM channel 1 linear factor and its fixed-step exponential
   DevBuf rows;                         // row index table of gnlse_copy_rows
   void* scratch = nullptr;             // this device's no-allocation workspace (device address)
+  int sched = 0;                       // schedule of the spectral-domain RK4IP launches (0: the grid's default)
 };
 
 // Fields per instance: 2 with the WDM channels, else 1.

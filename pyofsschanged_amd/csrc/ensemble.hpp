@@ -11,6 +11,10 @@
 // paths differ only in points per thread (N/T) and pass plan; T is chosen per
 // ensemble by a timed trial of the compiled candidates (512, 1024).
 //
+// Ensembles whose groups all carry the full Raman + self-steepening operator
+// without gain (rk4ip) run the spectral-domain RK4IP of the single-grid
+// launches (k_ensemble_sp), each grid with its default schedule.
+//
 // Groups are laid out in the order the host gives them (largest grid first),
 // so the workgroups dispatched first are the longest-running ones.
 //
@@ -64,6 +68,43 @@ __global__ void __launch_bounds__(T) k_ensemble(const EnsembleGroup* __restrict_
     adaptive_body<16384, T, METHOD, K>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws, g.spec);
 }
 
+// Schedule (Dev's CFG) of a grid's spectral-domain RK4IP inside an ensemble:
+// the lane-pair N-point plan of the 8192- and 16384-sample grids' own
+// launches where the points per thread allow it.
+template <int N, int T>
+constexpr int ens_sp_cfg() {
+  return (N >= 8192 && N / T <= 16) ? 8 : 0;
+}
+template <int T>
+struct EnsembleShapeSP {
+  static constexpr int a = sp_lds_doubles<4096, T, ens_sp_cfg<4096, T>()>();
+  static constexpr int b = sp_lds_doubles<8192, T, ens_sp_cfg<8192, T>()>();
+  static constexpr int c = sp_lds_doubles<16384, T, ens_sp_cfg<16384, T>()>();
+  static constexpr int LDS_DOUBLES = a > b ? (a > c ? a : c) : (b > c ? b : c);
+};
+
+template <int T>
+__global__ void __launch_bounds__(T) k_ensemble_sp(const EnsembleGroup* __restrict__ groups, int ngroups) {
+  __shared__ double lds[EnsembleShapeSP<T>::LDS_DOUBLES];
+  __shared__ double red[18 * (T / 64) + 2];
+  const long long w = blockIdx.x;
+  int gi = 0;
+#pragma unroll
+  for (int i = 1; i < kEnsMaxGroups; ++i)
+    if (i < ngroups && w >= groups[i].first) gi = i;
+  const EnsembleGroup& g = groups[gi];
+  const size_t b = (size_t)(w - g.first);
+  if (g.n == 4096)
+    adaptive_body<4096, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<4096, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+                                                                              g.ws, g.spec);
+  else if (g.n == 8192)
+    adaptive_body<8192, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<8192, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+                                                                              g.ws, g.spec);
+  else
+    adaptive_body<16384, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<16384, T>()>(lds, red, g.ops, g.prm, b, g.in,
+                                                                                g.out, g.ws, g.spec);
+}
+
 // g12 of one group: one thread per spectral sample, the realisations summed
 // in index order (reads are unit-stride across the wavefront for each i).
 template <int BS>
@@ -92,7 +133,8 @@ struct EnsembleLauncher {
   static hipError_t run(hipStream_t s, int method, int kind, const EnsembleGroup* groups, int ngroups,
                         long long blocks) {
     if (method == M_RK4IP) {
-      if (kind == NL_KERR) hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_KERR>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
+      if (kind == NL_ALL) hipLaunchKernelGGL((k_ensemble_sp<T>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
+      else if (kind == NL_KERR) hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_KERR>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
       else hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_GENERIC>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
     } else {
       if (kind == NL_KERR)

@@ -3,4 +3,7 @@
 #ifndef GN_T_1024
 #define GN_T_1024 128
 #endif
-GN_DEFINE_GRID(1024, GN_T_1024)
+#ifndef GN_C_1024
+#define GN_C_1024 0
+#endif
+GN_DEFINE_GRID(1024, GN_T_1024, GN_C_1024)

@@ -3,4 +3,7 @@
 #ifndef GN_T_2048
 #define GN_T_2048 256
 #endif
-GN_DEFINE_GRID(2048, GN_T_2048)
+#ifndef GN_C_2048
+#define GN_C_2048 0
+#endif
+GN_DEFINE_GRID(2048, GN_T_2048, GN_C_2048)

@@ -3,4 +3,7 @@
 #ifndef GN_T_256
 #define GN_T_256 64
 #endif
-GN_DEFINE_GRID(256, GN_T_256)
+#ifndef GN_C_256
+#define GN_C_256 0
+#endif
+GN_DEFINE_GRID(256, GN_T_256, GN_C_256)

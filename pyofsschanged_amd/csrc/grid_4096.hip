@@ -3,4 +3,7 @@
 #ifndef GN_T_4096
 #define GN_T_4096 512
 #endif
-GN_DEFINE_GRID(4096, GN_T_4096)
+#ifndef GN_C_4096
+#define GN_C_4096 0
+#endif
+GN_DEFINE_GRID(4096, GN_T_4096, GN_C_4096)

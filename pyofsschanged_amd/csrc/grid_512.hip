@@ -3,4 +3,7 @@
 #ifndef GN_T_512
 #define GN_T_512 64
 #endif
-GN_DEFINE_GRID(512, GN_T_512)
+#ifndef GN_C_512
+#define GN_C_512 0
+#endif
+GN_DEFINE_GRID(512, GN_T_512, GN_C_512)

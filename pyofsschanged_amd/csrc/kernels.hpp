@@ -7,7 +7,7 @@
 // batch of instances is one launch with one workgroup per instance.
 //
 // Each grid size is compiled in its own translation unit (grid_<N>.hip),
-// which instantiates Launcher<N, T> and exports its table.
+// which instantiates Launcher<N, T, C> and exports its table.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -67,9 +67,29 @@ struct LaunchTable {
   // peak flags (the other grids hold them in LDS)
   hipError_t (*characts)(hipStream_t, long long count, const double2* A, const double2* S, const double* w,
                          double* out, int* npk, int* pk, int max_pk, double2* scratch);
+  // schedule (Dev's CFG) of the spectral-domain RK4IP kernels in this table
+  int cfg;
 };
 
 const LaunchTable* find_table(long long n);
+
+// A compiled schedule of the spectral-domain adaptive RK4IP kernels of a grid
+// (full Raman + self-steepening operator, without gain or with the saturable
+// gain): threads per instance and Dev's CFG (transform plans, stage-pass
+// block). A grid's default schedule is its LaunchTable's; the autotuner
+// (gnlse_tune_adaptive) times the alternatives on the caller's inputs.
+struct Schedule {
+  int t, cfg;
+  hipError_t (*adaptive)(hipStream_t, const Ops&, const AdaptiveParams&, long long batch, const double2* in,
+                         double2* out, double2* ws);
+};
+// The launches a schedule covers: rk4ip with the full nonlinear operator and
+// no gain (SP 1) or the saturable gain with the flat interior Simpson weight
+// (SP 2), single-channel fields.
+inline bool schedule_applies(const Ops& o, int method) {
+  return method == M_RK4IP && !o.wdm && o.nl_kind == NL_ALL &&
+         (o.gain == GAIN_NONE || (o.gain == GAIN_SATURABLE && o.simp_flat));
+}
 
 #define COMMA ,
 #define GN_SHARED_G(N, T, K, YB)                 \
@@ -173,14 +193,14 @@ __global__ void __launch_bounds__(T)
 // stepper.py:164-222 with the cached exp(factor*h/2) in ops.efix. SP as for
 // k_adaptive: rk4ip_sp with gain SP - 1, the field's spectrum carried from
 // step to step in S_T.
-template <int N, int T, int METHOD, int K, bool YB, int SP = 0>
+template <int N, int T, int METHOD, int K, bool YB, int SP = 0, int CFG = 0>
 __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const double2* __restrict__ in,
                                              double2* __restrict__ out, double2* ws_all) {
   static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
                 "spectral-domain stepping: rk4ip, NL_ALL");
-  __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
+  __shared__ double lds[SP ? sp_lds_doubles<N, T, CFG>() : Dev<N, T, K, YB, CFG>::F::LDS_DOUBLES];
   __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
-  Dev<N, T, K, YB> d(lds, red, ops);
+  Dev<N, T, K, YB, CFG> d(lds, red, ops);
   constexpr int P = N / T;
   const size_t b = blockIdx.x;
   double2* ws = ws_all + b * NSLOT * N;
@@ -219,13 +239,13 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // (SpGain: 1 none, 2 saturable, 3 two-level Yb): the accepted field is then
 // also kept as its spectrum in S_T and the half step's spectrum, left in
 // S_HS, starts the fine step.
-template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0>
+template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0, int CFG = 0>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
                                               double2* ws_all, double2* __restrict__ spec) {
   static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
                 "spectral-domain stepping: rk4ip, NL_ALL");
-  Dev<N, T, K, YB> d(lds, red, ops);
+  Dev<N, T, K, YB, CFG> d(lds, red, ops);
   constexpr int P = N / T;
   double2* ws = ws_all + b * NSLOT * N;
   double2* sA = ws + S_A * N;
@@ -397,12 +417,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
 }
 
-template <int N, int T, int METHOD, int K, bool YB, int SP = 0>
+template <int N, int T, int METHOD, int K, bool YB, int SP = 0, int CFG = 0>
 __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                 double2* __restrict__ out, double2* ws_all) {
-  __shared__ double lds[SP ? sp_lds_doubles<N, T>() : FFT<N, T>::LDS_DOUBLES];
+  __shared__ double lds[SP ? sp_lds_doubles<N, T, CFG>() : Dev<N, T, K, YB, CFG>::F::LDS_DOUBLES];
   __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
-  adaptive_body<N, T, METHOD, K, YB, SP>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
+  adaptive_body<N, T, METHOD, K, YB, SP, CFG>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
@@ -432,7 +452,10 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
 
 namespace gn {
 
-template <int N, int T>
+// C is the schedule (Dev's CFG) of the spectral-domain RK4IP kernels (the
+// launches with the full Raman + self-steepening operator); the other
+// kernels of the grid use the plain one.
+template <int N, int T, int C = 0>
 struct Launcher {
   static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out,
                         double2*) {
@@ -503,11 +526,11 @@ struct Launcher {
     else if (o.wdm)
       hipLaunchKernelGGL((k_fixed_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_YB)
-      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_NONE)
-      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_SATURABLE && o.simp_flat)
-      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1 + SPG_SAT>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+      hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1 + SPG_SAT, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                          ws);
     else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
@@ -530,11 +553,11 @@ struct Launcher {
     else if (o.wdm)
       hipLaunchKernelGGL((k_adaptive_wdm<N, T, M_SS_SYMMETRIC>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.gain == GAIN_NONE && o.nl_kind == NL_ALL)
-      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.gain == GAIN_SATURABLE && o.simp_flat && o.nl_kind == NL_ALL)
-      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 2>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 2, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (method == M_RK4IP && o.gain == GAIN_YB && o.nl_kind == NL_ALL)
-      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                          ws);
     else if (method == M_RK4IP)
       GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
@@ -547,12 +570,31 @@ struct Launcher {
     return hipGetLastError();
   }
   static constexpr LaunchTable table = {N,       T,     NSLOT, 1,           &fft,  &make_exp, &linear,  &linfac,
-                                        &nonlin, &exp_nonlin, &step, &fixed, &adaptive, &characts};
+                                        &nonlin, &exp_nonlin, &step, &fixed, &adaptive, &characts, C};
+};
+
+template <int N, int T, int C>
+struct SchedLauncher {
+  static hipError_t adaptive(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
+                             double2* out, double2* ws) {
+    if (o.gain == GAIN_NONE)
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (o.gain == GAIN_SATURABLE && o.simp_flat)
+      hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 2, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else return hipErrorNotSupported;
+    return hipGetLastError();
+  }
+  static constexpr Schedule sched = {T, C, &adaptive};
 };
 
 }  // namespace gn
 
-#define GN_DEFINE_GRID(NN, TT) \
-  namespace gn {               \
-  const LaunchTable* table_##NN() { return &Launcher<NN, TT>::table; } \
+#define GN_DEFINE_SCHEDULE(NN, TT, CC, ID) \
+  namespace gn {                            \
+  const Schedule* schedule_##NN##_##ID() { return &SchedLauncher<NN, TT, CC>::sched; } \
+  }
+
+#define GN_DEFINE_GRID(NN, TT, CC) \
+  namespace gn {                   \
+  const LaunchTable* table_##NN() { return &Launcher<NN, TT, CC>::table; } \
   }

@@ -246,9 +246,18 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
 // the other operators and their register live ranges.
 // YB compiles the two-level Yb gain into the linear operator (its branch
 // costs registers, so kernels without it do not carry it).
-template <int N, int T, int KIND, bool YB = false>
+// CFG is the compiled schedule (the autotuner picks among compiled ones):
+// bits 0-3 the plan policy of the N-point transform (FFT's PL), bits 4-7
+// that of the N/2-point transform of the Raman convolution, bits 8-9 the
+// block of the stage passes of rk4ip_sp (0: 4 points, 1: 2, 2: 8). A
+// lane-pair plan in either transform renumbers the threads of the whole
+// kernel (every transform and pass follows one ownership rule).
+template <int N, int T, int KIND, bool YB = false, int CFG = 0>
 struct Dev {
-  using F = FFT<N, T>;
+  static constexpr int RENUM = ((CFG & 8) || (CFG & 0x80)) ? 16 : 0;
+  using F = FFT<N, T, (CFG & 15) | RENUM>;
+  using FH = FFT<N / 2, T, ((CFG >> 4) & 15) | RENUM>;
+  static constexpr int TILE = ((CFG >> 8) & 3) == 1 ? 2 : ((CFG >> 8) & 3) == 2 ? 8 : 4;
   static constexpr int P = N / T;
   static constexpr double INVN = 1.0 / N;  // exact: N is a power of two
   static constexpr int NW = T / 64;
@@ -263,11 +272,12 @@ struct Dev {
   // Owned index of register k. The thread index goes through opaque() so
   // that slot addresses are formed at each use instead of being hoisted out
   // of the step loops and kept live for the whole kernel.
-  __device__ __forceinline__ unsigned idx(int k) const { return (unsigned)opaque((int)threadIdx.x) + T * k; }
+  __device__ __forceinline__ unsigned tid() const { return (unsigned)opaque(F::tid()); }
+  __device__ __forceinline__ unsigned idx(int k) const { return tid() + T * k; }
   // The thread's first owned point of an array; its other points follow at
   // strides of T (the pointwise helpers below index it so).
   template <class V>
-  __device__ __forceinline__ V* mine(V* p) const { return p + (unsigned)opaque((int)threadIdx.x); }
+  __device__ __forceinline__ V* mine(V* p) const { return p + tid(); }
   __device__ __forceinline__ int kind() const { return KIND == NL_GENERIC ? op.nl_kind : KIND; }
 
   __device__ __forceinline__ void load(double2 (&x)[P], const double2* __restrict__ g) const {
@@ -563,8 +573,8 @@ enum Slot { S_A = 0, S_NA, S_H, S_AI, S_K0, S_K1, S_EQ, S_EH, S_T, S_U, S_HS, NS
 // from nA instead of being computed when nA_ready, and stored to nA when
 // nA_store (the step-doubling controller evaluates N of the same field for
 // its half step and its coarse step).
-template <int N, int T, int K, bool YB>
-__device__ __forceinline__ void rk4ip(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void rk4ip(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
                       double2* ws, double2* nA = nullptr, bool nA_ready = false, bool nA_store = false) {
   constexpr int P = N / T;
   double2* sAI = ws + S_AI * N;
@@ -670,12 +680,12 @@ __device__ __forceinline__ double gain_factor(double2 q, double gp, double hl) {
 // Entry of rk4ip_sp with a gain: A_I~ = factor E A~ into AIs. For SPG_SAT
 // the exponent g0 comes from A's time samples (the caller); for SPG_YB the
 // inversion comes from the spectrum A~ itself. x is the load buffer.
-template <int N, int T, int GK, int C, int KIND, bool YB>
-__device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], const double2* E, bool esq,
+template <int N, int T, int GK, int C, int KIND, bool YB, int CFG>
+__device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB, CFG>& d, double2 (&x)[N / T], const double2* E, bool esq,
                                               const double2* As, double2* AIs, double g0, double hl, double* red) {
   constexpr int P = N / T;
   const Ops& op = d.op;
-  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+  const unsigned t = d.tid();
 #pragma unroll
   for (int k = 0; k < P; ++k) x[k] = As[T * k];
   double gp = g0;
@@ -710,15 +720,15 @@ __device__ __forceinline__ void pw_sp_entry_g(Dev<N, T, KIND, YB>& d, double2 (&
 //           the reference's call order: after L(A_I + k2)).
 //   ST = 4: x <- conj(k3~/6 + factor(gp4) E S), A'~ also to out.
 // red holds 2 SpSums<GK>::M NW doubles.
-template <int N, int T, int ST, int C, int GK, int KIND, bool YB>
-__device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&x)[N / T], bool fresh, double2* nA,
+template <int N, int T, int ST, int C, int GK, int KIND, bool YB, int CFG>
+__device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB, CFG>& d, double2 (&x)[N / T], bool fresh, double2* nA,
                                               double h, double hl, const double2* E, bool esq, double2* S,
                                               const double2* AIs, double2* out, double& gp4, double* red) {
   constexpr int P = N / T;
   constexpr int M = SpSums<GK>::M;
   constexpr double INVN = 1.0 / N;
   const Ops& op = d.op;
-  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+  const unsigned t = d.tid();
   if constexpr (ST == 1) {
     double a[M] = {};
 #pragma unroll
@@ -810,9 +820,9 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB>& d, double2 (&
 
 // LDS (doubles) of a kernel that runs rk4ip_sp: the N-point transform, the
 // N/2-point one and the N real values of the Raman exchanges.
-template <int N, int T>
+template <int N, int T, int CFG = 0>
 constexpr int sp_lds_doubles() {
-  constexpr int a = FFT<N, T>::LDS_DOUBLES, b = FFT<N / 2, T>::LDS_DOUBLES;
+  constexpr int a = Dev<N, T, NL_ALL, false, CFG>::F::LDS_DOUBLES, b = Dev<N, T, NL_ALL, false, CFG>::FH::LDS_DOUBLES;
   constexpr int m = a > b ? a : b;
   return m > N ? m : N;
 }
@@ -828,21 +838,21 @@ constexpr int sp_lds_doubles() {
 // transform of a real response) and is dropped. On entry the LDS holds p
 // (raman_stage); on exit x_k holds the convolution at owned point k (real
 // part; imaginary part 0).
-template <int N, int T, int K, bool YB>
-__device__ __forceinline__ void raman_stage(Dev<N, T, K, YB>& d, const double2 (&x)[N / T]) {
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void raman_stage(Dev<N, T, K, YB, CFG>& d, const double2 (&x)[N / T]) {
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < N / T; ++k) d.lds[d.idx(k)] = abs2(x[k]);
 }
-template <int N, int T, int K, bool YB>
-__device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N / T]) {
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void raman_real(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T]) {
   constexpr int M = N / 2, P = N / T, P2 = M / T;
-  using F2 = FFT<M, T>;
+  using F2 = typename Dev<N, T, K, YB, CFG>::FH;
   constexpr double INVN = 1.0 / N;
   const Ops& op = d.op;
   double* ld = d.lds;
   double2* l2 = reinterpret_cast<double2*>(d.lds);
-  const unsigned t = (unsigned)opaque((int)threadIdx.x);
+  const unsigned t = d.tid();
   typename F2::Twiddles tw2;
   F2::template load_twiddles<2>(tw2, op.tw);
   __syncthreads();
@@ -853,7 +863,7 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
   for (int it = 0; it < 2; ++it) {
     F2::plus(z, d.lds, tw2);
     if (it == 0) {  // pair Z_k with Z_M-k, multiply, fold back to M points
-      const unsigned u = (unsigned)opaque((int)threadIdx.x);
+      const unsigned u = d.tid();
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < P2; ++j) l2[u + T * j] = z[j];
@@ -910,20 +920,21 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB>& d, double2 (&x)[N /
 // hold); SPG_YB, the two-level Yb gain of amplifier.py:368-402 (from the
 // spectra of those arrays; the pump power advances in the reference's call
 // order). The workgroup's red buffer holds 18 NW doubles.
-template <int GK, int N, int T, int K, bool YB>
-__device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
+template <int GK, int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in, const double2* inS,
                                          double2* outS, const double2* E, bool esq, double h, double2* ws, double2* nA,
                                          bool nA_ready, bool nA_store) {
   static_assert(K == NL_ALL && YB == (GK == SPG_YB), "spectral-domain rk4ip: full nonlinear operator");
-  using F = FFT<N, T>;
+  using F = typename Dev<N, T, K, YB, CFG>::F;
   constexpr int P = N / T;
-  constexpr double INVN = Dev<N, T, K, YB>::INVN;
+  constexpr double INVN = Dev<N, T, K, YB, CFG>::INVN;
   double2* sAIs = ws + S_K0 * N;  // A_I~
   double2* sS = ws + S_K1 * N;    // k0~, then k0~ + 2 k1~, then the bracket of A'~
   double2* sU = ws + S_U * N;     // the field x of the running N(x)
   const Ops& op = d.op;
   const double a1 = 1.0 - op.f_R;
-  constexpr int CW = P < 4 ? P : 4;
+  constexpr int TL = Dev<N, T, K, YB, CFG>::TILE;
+  constexpr int CW = P < TL ? P : TL;
   int p0 = 0;
   double gp4 = 0.0;  // gain parameter of L(acc), from stage 3
   if constexpr (GK != SPG_NONE) {
@@ -1000,8 +1011,8 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB>& d, double2 (&x)[N / T
 
 // solver.py:266-275 ss_symmetric: L(h/2), exp_non with the phase of the
 // un-propagated input, L(h/2).
-template <int N, int T, int K, bool YB>
-__device__ __forceinline__ void ss_symmetric(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void ss_symmetric(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in, const double2* E, double h,
                              double2* ws) {
   double2* sT = ws + S_T * N;
   double2* sU = ws + S_U * N;
@@ -1018,8 +1029,8 @@ __device__ __forceinline__ void ss_symmetric(Dev<N, T, K, YB>& d, double2 (&x)[N
 // (k_j = h N(A_L + c_j k_(j-1)), A_N = A_L + (k0 + 2 (k1 + k2) + k3)/6) on the
 // nonlinear operator alone, L(h/2); opencl_fibre.py:929-970 cl_ss_sym_rk4 is
 // the same scheme. One nonlinear call site in a 4-stage loop.
-template <int N, int T, int K, bool YB>
-__device__ __forceinline__ void ss_sym_rk4(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in,
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void ss_sym_rk4(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in,
                                            const double2* E, double h, double2* ws) {
   constexpr int P = N / T;
   double2* sAL = ws + S_AI * N;
@@ -1058,8 +1069,8 @@ __device__ __forceinline__ void ss_sym_rk4(Dev<N, T, K, YB>& d, double2 (&x)[N /
   d.linear(x, E, hh);
 }
 
-template <int METHOD, int N, int T, int K, bool YB>
-__device__ __forceinline__ void integrator(Dev<N, T, K, YB>& d, double2 (&x)[N / T], const double2* in, const double2* E,
+template <int METHOD, int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void integrator(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in, const double2* E,
                                            double h, double2* ws, double2* nA = nullptr, bool nA_ready = false,
                                            bool nA_store = false) {
   if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);

@@ -1,0 +1,5 @@
+// Schedule 2 of the 16384-sample grid's spectral-domain adaptive RK4IP
+// kernels, one of the autotuner's candidates: 1024 threads, lane-pair passes
+// in both transforms.
+#include "kernels.hpp"
+GN_DEFINE_SCHEDULE(16384, 1024, 0x88, 2)

@@ -39,6 +39,47 @@ def test_library_exports_every_declared_function():
     assert not _lib.grid_supported(131072)
 
 
+def _radices(plan):
+    return [(int(r.rstrip("L")), r.endswith("L")) for r in plan.split(".")]
+
+
+def test_schedules_describe_valid_plans():
+    """Every compiled schedule of the spectral-domain RK4IP kernels: its
+    N-point and N/2-point pass plans factor the transform length, every
+    in-thread radix divides the points per thread P and every lane-pair pass
+    has radix 2P (two lanes of P points each); schedule 0 is the grid's
+    default launch shape."""
+    from pyofsschanged_amd import _lib
+    for n in (4096, 8192, 16384):
+        sc = _lib.schedules(n)
+        assert len(sc) >= 2 and [s["index"] for s in sc] == list(range(len(sc)))
+        descs = set()
+        for s in sc:
+            t_part, n_part, r_part, tile = s["desc"].split(" ")
+            T = int(t_part[1:])
+            assert T == s["threads"] and T % 64 == 0 and tile.startswith("tile")
+            for length, plan in ((n, n_part[2:]), (n // 2, r_part[2:])):
+                P = length // T
+                rad = _radices(plan)
+                assert int(np.prod([r for r, _ in rad])) == length, s["desc"]
+                for r, lane in rad:
+                    assert (r == 2 * P) if lane else (P % r == 0), s["desc"]
+            descs.add(s["desc"])
+        assert len(descs) == len(sc)  # no duplicate schedule
+    # defaults: a lane-pair pass in the 8192- and 16384-point transforms;
+    # schedule 1 keeps the plain plans
+    assert _lib.schedules(8192)[0]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile4"
+    assert _lib.schedules(8192)[1]["desc"] == "T512 N:16.16.2.16 R:8.8.8.8 tile4"
+    assert _lib.schedules(16384)[0]["desc"].startswith("T1024 N:32L.32L.16 ")
+    assert _lib.schedules(16384)[1]["desc"].startswith("T1024 N:16.16.4.16 ")
+    assert _lib.schedules(4096)[0]["desc"] == "T512 N:8.8.8.8 R:4.4.4.4.2.4 tile4"
+    for n in (256, 1024):
+        assert len(_lib.schedules(n)) == 1
+    assert _lib.schedules(32768)[0]["desc"] == "T1024 two-level"
+    with pytest.raises(_lib.GNLSEError):
+        _lib.schedules(1000)
+
+
 def test_struct_layouts_match_header():
     from pyofsschanged_amd import _lib
     assert ctypes.sizeof(_lib.AdaptiveParams) == 7 * 8 + 3 * 8 + 2 * 4

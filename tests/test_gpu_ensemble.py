@@ -200,3 +200,58 @@ def test_noise_ensemble_full_size(W):
             print(f"  N=4096 r=0 full length vs oracle: attempts {att[0]} ({n_att}), rel L2 {err:.2e}", flush=True)
             assert att[0] == n_att
             assert err <= bar
+
+
+@pytest.mark.parametrize("threads", [512, 1024])
+def test_supercontinuum_ensemble_matches_oracle(W, threads):
+    """Groups with the full Raman + self-steepening operator (the C2
+    supercontinuum physics) run the spectral-domain RK4IP in the ensemble
+    launch: every realisation (sech + one-photon-per-mode noise, mixed
+    grids) against the oracle's Fibre run, identical attempt count and
+    relative L2 <= 1e-9, and g12 from the device's own spectra. Where the
+    oracle raises SmallStepSizeError (stepper.py:349-351; the noisy
+    16384-sample input here), the realisation ends with that status."""
+    from pyofsschanged_amd import _lib
+    from helpers import DUDLEY_BETA, engine_context, oracle_ops
+    groups = ((8192, 2), (4096, 2), (16384, 1))
+    length = 3e-6
+    ctxs, inputs, doms = [], [], []
+    seed = 2000
+    for N, M in groups:
+        dom = O.Domain(1, N, 12.5, C_NM_PS / 835.0)
+        A0 = O.Sech(peak_power=1e4, width=0.050, using_fwhm=True).generate(dom.t)
+        inputs.append(np.stack([A0 + O.oppm_noise(dom, seed + r) for r in range(M)]))
+        seed += M
+        lin, nl = oracle_ops(dom, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True)
+        ctxs.append(engine_context(dom, lin, nl, 1))
+        doms.append(dom)
+    p = _lib.AdaptiveParams()
+    p.length, p.h0, p.local_error = length, min(length / 100, 1e-6), 1e-6
+    p.safety, p.max_factor, p.min_factor, p.step_size_min = 0.9, 10.0, 0.2, 1e-37
+    p.traces, p.total_attempts, p.steps_max, p.eta = 1, 1000, 100000, 5
+    ens = _lib.Ensemble(0, p, [(c, M) for c, (_, M) in zip(ctxs, groups)])
+    for g in range(len(groups)):
+        ens.set_input(g, inputs[g])
+    assert ens.tune(threads) == threads
+    ms = ens.run()
+    print(f"supercontinuum ensemble {groups} at {threads} threads: {ms:.1f} ms", flush=True)
+    for g, (N, M) in enumerate(groups):
+        out = ens.output(g)
+        for r in range(M):
+            f = O.Fibre(length=length, beta=DUDLEY_BETA, gamma=110.0, self_steepening=True, use_all=True,
+                        method="ark4ip", local_error=1e-6)
+            res = out["results"][r]
+            try:
+                ref = f(doms[g], inputs[g][r].copy())
+            except O.SmallStepSizeError:
+                print(f"  N={N} r={r}: oracle SmallStepSizeError, status {res['status']}", flush=True)
+                assert res["status"] == _lib.STATUS_SMALL_STEP
+                continue
+            err = rel_l2(out["A"][r], ref)
+            n_att = len(f.stepper.attempt_log)
+            print(f"  N={N} r={r}: attempts {res['n_attempts']} (oracle {n_att}), rel L2 {err:.2e}", flush=True)
+            assert res["status"] == 0
+            assert res["n_attempts"] == n_att
+            assert err <= FIELD_TOL
+        if M > 1:
+            assert np.max(np.abs(out["g12"] - O.coherence_g12(O.fft(out["A"])))) < 1e-9

@@ -1,0 +1,137 @@
+"""GPU parity of every compiled schedule of the spectral-domain adaptive
+RK4IP kernels (threads per instance, pass plans of the N-point and N/2-point
+transforms including lane-pair passes, block of the stage passes) against
+the oracle, and the autotuner that picks among them.
+
+Bar (as tests/test_gpu_propagation.py): the oracle's attempt sequence
+(identical accept/reject decisions and count, step sizes to 1e-6 relative)
+and output field relative L2 <= 1e-9. Workloads: the C2 supercontinuum
+physics (Raman + self-steepening, no gain) over the first 1e-5 km (2e-6 km
+at 16384 samples), and the C4 amplifier (saturable gain) over 5e-4 km at
+8192 and 2e-4 km at 16384 samples.
+"""
+
+import functools
+
+import numpy as np
+import pytest
+
+from helpers import amplifier_c4, engine_context, oracle_amplifier, oracle_ops, rel_l2, supercontinuum_c2
+from test_gpu_propagation import adaptive_oracle, adaptive_params
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-9
+GRIDS = (4096, 8192, 16384)
+
+
+def _schedule_cases():
+    try:
+        from pyofsschanged_amd import _lib
+        return [(n, s["index"]) for n in GRIDS for s in _lib.schedules(n)]
+    except Exception:  # library not built: one case, which fails loudly when run
+        return [(n, 0) for n in GRIDS]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from pyofsschanged_amd import _lib
+    if _lib.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return _lib
+
+
+@functools.lru_cache(maxsize=None)
+def c2_reference(N):
+    dom, A, fib = supercontinuum_c2(N)
+    fib["length"] = 2e-6 if N == 16384 else 1e-5
+    ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+    return dom, A, fib, ref, f.stepper.attempt_log
+
+
+@functools.lru_cache(maxsize=None)
+def c4_reference(N):
+    dom, A, fib = amplifier_c4(N, 0.5)
+    fib["length"] = 2e-4 if N == 16384 else 5e-4
+    ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+    return dom, A, fib, ref, f.stepper.attempt_log
+
+
+def check_against_oracle(lib, ctx, A, fib, ref, log, tag):
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6), log_cap=len(log) + 8)
+    r = out["results"][0]
+    err = rel_l2(out["A"], ref)
+    ref_h = np.array([a[0] for a in log])
+    ref_acc = np.array([a[2] for a in log], dtype=np.uint8)
+    n = min(len(log), int(r["n_attempts"]))
+    dh = np.max(np.abs(out["log_h"][0, :n] / ref_h[:n] - 1))
+    print(f"{tag}: attempts {r['n_attempts']} (oracle {len(log)}), rel L2 {err:.3e}, dh {dh:.1e}, "
+          f"kernel {ctx.last_kernel_ms():.1f} ms", flush=True)
+    assert r["status"] == 0
+    assert r["n_attempts"] == len(log)
+    assert np.array_equal(out["log_accept"][0, :len(log)], ref_acc)
+    assert dh < 1e-6
+    assert err <= FIELD_TOL
+    return out
+
+
+@pytest.mark.parametrize("N,index", _schedule_cases())
+def test_schedule_c2_against_oracle(lib, N, index):
+    """Raman + self-steepening without gain (the headline's kernel)."""
+    dom, A, fib, ref, log = c2_reference(N)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    desc = lib.schedules(N)[index]["desc"]
+    ctx.set_schedule(index)
+    assert ctx.get_schedule() == index
+    check_against_oracle(lib, ctx, A, fib, ref, log, f"C2 N={N} schedule {index} ({desc})")
+
+
+@pytest.mark.parametrize("N,index", [c for c in _schedule_cases() if c[0] in (8192, 16384)])
+def test_schedule_c4_against_oracle(lib, N, index):
+    """The saturable-gain amplifier (gain energies reduced over the
+    workgroup inside the stage passes)."""
+    dom, A, fib, ref, log = c4_reference(N)
+    amp = oracle_amplifier(dom, fib["length"], **fib["amplifier"])
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], use_all=True, amplifier=amp)
+    ctx = engine_context(dom, lin, nl, 1, amplifier=amp)
+    ctx.set_schedule(index)
+    check_against_oracle(lib, ctx, A, fib, ref, log, f"C4 N={N} schedule {index}")
+
+
+def test_autotuner_picks_a_schedule_and_keeps_it(lib):
+    """gnlse_tune_adaptive times every schedule, selects the fastest for the
+    context and leaves it selected: the next propagation is bit for bit the
+    one of that schedule set by hand, and still the oracle's."""
+    N = 8192
+    dom, A, fib, ref, log = c2_reference(N)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    params = adaptive_params(lib, fib, 1e-6)
+    batch = np.stack([A, A * (1 + 1e-3), A * (1 - 1e-3), A])
+    chosen, ms = ctx.tune_adaptive(batch, params, trial_attempts=20, repeats=2)
+    count = len(lib.schedules(N))
+    print(f"tuner: chose {chosen} of {count}; trial ms {[round(m, 3) for m in ms]}", flush=True)
+    assert 0 <= chosen < count
+    assert len(ms) == count and all(m > 0 for m in ms)
+    assert ms[chosen] == min(ms)
+    assert ctx.get_schedule() == chosen
+    tuned = check_against_oracle(lib, ctx, A, fib, ref, log, f"C2 N={N} tuned schedule {chosen}")
+    ctx2 = engine_context(dom, lin, nl, 1)
+    ctx2.set_schedule(chosen)
+    by_hand = ctx2.propagate_adaptive(0, A, params, log_cap=len(log) + 8)
+    assert np.array_equal(tuned["A"], by_hand["A"])
+
+
+def test_schedule_selection_errors(lib):
+    dom, A, fib, ref, log = c2_reference(4096)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)  # Kerr only: no schedules cover it
+    with pytest.raises(lib.GNLSEError):
+        ctx.set_schedule(len(lib.schedules(4096)))
+    with pytest.raises(lib.GNLSEError, match="no schedules"):
+        ctx.tune_adaptive(A, adaptive_params(lib, fib, 1e-6))
+    # a new grid starts from its default schedule
+    ctx.set_schedule(1)
+    ctx.set_grid(8192)
+    assert ctx.get_schedule() == 0
