@@ -41,11 +41,19 @@ struct EnsembleGroup {
   long long n, first, count;
 };
 
+// Schedule (Dev's CFG) of a grid inside an ensemble launch: the lane-pair
+// N-point plan of the 8192- and 16384-sample grids' own launches where the
+// points per thread allow it.
+template <int N, int T>
+constexpr int ens_cfg() {
+  return (N >= 8192 && N / T <= 16) ? 8 : 0;
+}
+
 template <int T>
 struct EnsembleShape {
-  static constexpr int a = FFT<4096, T>::LDS_DOUBLES;
-  static constexpr int b = FFT<8192, T>::LDS_DOUBLES;
-  static constexpr int c = FFT<16384, T>::LDS_DOUBLES;
+  static constexpr int a = Dev<4096, T, NL_KERR, false, ens_cfg<4096, T>()>::F::LDS_DOUBLES;
+  static constexpr int b = Dev<8192, T, NL_KERR, false, ens_cfg<8192, T>()>::F::LDS_DOUBLES;
+  static constexpr int c = Dev<16384, T, NL_KERR, false, ens_cfg<16384, T>()>::F::LDS_DOUBLES;
   static constexpr int LDS_DOUBLES = a > b ? (a > c ? a : c) : (b > c ? b : c);
 };
 
@@ -61,25 +69,21 @@ __global__ void __launch_bounds__(T) k_ensemble(const EnsembleGroup* __restrict_
   const EnsembleGroup& g = groups[gi];
   const size_t b = (size_t)(w - g.first);
   if (g.n == 4096)
-    adaptive_body<4096, T, METHOD, K>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws, g.spec);
+    adaptive_body<4096, T, METHOD, K, false, 0, ens_cfg<4096, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
+                                                                     g.spec);
   else if (g.n == 8192)
-    adaptive_body<8192, T, METHOD, K>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws, g.spec);
+    adaptive_body<8192, T, METHOD, K, false, 0, ens_cfg<8192, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
+                                                                     g.spec);
   else
-    adaptive_body<16384, T, METHOD, K>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws, g.spec);
+    adaptive_body<16384, T, METHOD, K, false, 0, ens_cfg<16384, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+                                                                       g.ws, g.spec);
 }
 
-// Schedule (Dev's CFG) of a grid's spectral-domain RK4IP inside an ensemble:
-// the lane-pair N-point plan of the 8192- and 16384-sample grids' own
-// launches where the points per thread allow it.
-template <int N, int T>
-constexpr int ens_sp_cfg() {
-  return (N >= 8192 && N / T <= 16) ? 8 : 0;
-}
 template <int T>
 struct EnsembleShapeSP {
-  static constexpr int a = sp_lds_doubles<4096, T, ens_sp_cfg<4096, T>()>();
-  static constexpr int b = sp_lds_doubles<8192, T, ens_sp_cfg<8192, T>()>();
-  static constexpr int c = sp_lds_doubles<16384, T, ens_sp_cfg<16384, T>()>();
+  static constexpr int a = sp_lds_doubles<4096, T, ens_cfg<4096, T>()>();
+  static constexpr int b = sp_lds_doubles<8192, T, ens_cfg<8192, T>()>();
+  static constexpr int c = sp_lds_doubles<16384, T, ens_cfg<16384, T>()>();
   static constexpr int LDS_DOUBLES = a > b ? (a > c ? a : c) : (b > c ? b : c);
 };
 
@@ -95,13 +99,13 @@ __global__ void __launch_bounds__(T) k_ensemble_sp(const EnsembleGroup* __restri
   const EnsembleGroup& g = groups[gi];
   const size_t b = (size_t)(w - g.first);
   if (g.n == 4096)
-    adaptive_body<4096, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<4096, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+    adaptive_body<4096, T, M_RK4IP, NL_ALL, false, 1, ens_cfg<4096, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
                                                                               g.ws, g.spec);
   else if (g.n == 8192)
-    adaptive_body<8192, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<8192, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+    adaptive_body<8192, T, M_RK4IP, NL_ALL, false, 1, ens_cfg<8192, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
                                                                               g.ws, g.spec);
   else
-    adaptive_body<16384, T, M_RK4IP, NL_ALL, false, 1, ens_sp_cfg<16384, T>()>(lds, red, g.ops, g.prm, b, g.in,
+    adaptive_body<16384, T, M_RK4IP, NL_ALL, false, 1, ens_cfg<16384, T>()>(lds, red, g.ops, g.prm, b, g.in,
                                                                                 g.out, g.ws, g.spec);
 }
 

@@ -238,7 +238,12 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // SP selects rk4ip_sp (RK4IP, full nonlinear operator) with gain SP - 1
 // (SpGain: 1 none, 2 saturable, 3 two-level Yb): the accepted field is then
 // also kept as its spectrum in S_T and the half step's spectrum, left in
-// S_HS, starts the fine step.
+// S_HS, starts the fine step. Without a gain (SP = 1) the workspace an
+// attempt touches is cut to 7 slots (so that a batch of 256 at 8192 samples,
+// 224 MiB, fits the 256 MiB Infinity Cache): the accepted field lives in S_U
+// (read only by the first call after an acceptance, before that call parks
+// its first stage field there) and the fine step forms the half step's
+// spectrum from its time samples instead of reading it from S_HS.
 template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0, int CFG = 0>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
@@ -253,10 +258,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   double2* sEQ = ws + S_EQ * N;
   double2* sEH = ws + S_EH * N;
   double2* sAs = ws + S_T * N;   // SP: fft(A)
-  double2* sHs = ws + S_HS * N;  // SP: fft of the half step's result
+  double2* sHs = SP == 1 ? nullptr : ws + S_HS * N;  // SP > 1: fft of the half step's result
+  double2* sAt = SP == 1 ? ws + S_U * N : sA;         // the accepted field (time domain)
   double2 x[P];
   d.load(x, in + b * N);
-  d.store(sA, x);
+  d.store(sAt, x);
   d.pump_begin(b);
 
   double2* sNA = ws + S_NA * N;
@@ -309,7 +315,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       // result replaces the half-step field in sH; the coarse one stays in x.
 #pragma unroll 1
       for (int c = 0; c < 3; ++c) {
-        const double2* src = (c == 1) ? sH : sA;
+        const double2* src = (c == 1) ? sH : sAt;
         const double2* E = (c == 2) ? sEH : sEQ;
         // N(A) is shared by the half and the coarse step, and by every
         // attempt from the same accepted field
@@ -369,7 +375,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
           for (int k = 0; k < P; ++k)
             x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
         }
-        d.store(sA, x);
+        d.store(sAt, x);
         nA_valid = false;
         if (z > z_ignore) {  // stepper.py:342-345
           if (prm.max_traces > 0 && !store_trace(z, h_temp)) status = GNLSE_STATUS_TRACE_OVERFLOW;
@@ -398,7 +404,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
   if (!finished && status == GNLSE_STATUS_OK) status = GNLSE_STATUS_MAX_STEPS;
   d.pump_end(b);
-  d.load(x, sA);
+  if (SP == 1 && !finished) {  // S_U may hold a stage field: the accepted field from its spectrum
+    d.load(x, sAs);
+    d.ifft(x);
+  } else {
+    d.load(x, sAt);
+  }
   d.store(out + b * N, x);
   if (spec) {
     d.fft(x);
@@ -446,15 +457,28 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
     else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false>), __VA_ARGS__);                                 \
   } while (0)
 
+// The same with the schedule CC (Dev's CFG) of the grid: the propagating
+// kernels (fixed and adaptive steppers) run their transforms on the grid's
+// plan.
+#define GN_BY_KIND_GAIN_C(ops, KERNEL, CC, ...)                                                          \
+  do {                                                                                                  \
+    if ((ops).gain == GAIN_YB) {                                                                        \
+      if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, true, 0, CC>), __VA_ARGS__);     \
+      else hipLaunchKernelGGL((KERNEL, NL_GENERIC, true, 0, CC>), __VA_ARGS__);                         \
+    } else if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, CC>), __VA_ARGS__); \
+    else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, false, 0, CC>), __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false, 0, CC>), __VA_ARGS__);                          \
+  } while (0)
+
 }  // namespace gn
 
 #include "wdm.hpp"
 
 namespace gn {
 
-// C is the schedule (Dev's CFG) of the spectral-domain RK4IP kernels (the
-// launches with the full Raman + self-steepening operator); the other
-// kernels of the grid use the plain one.
+// C is the schedule (Dev's CFG) of the grid's propagating kernels (fixed and
+// adaptive steppers, every method and operator); the single-operator
+// kernels and the WDM kernels use the plain one.
 template <int N, int T, int C = 0>
 struct Launcher {
   static hipError_t fft(hipStream_t s, const Ops& o, int sign, long long batch, const double2* in, double2* out,
@@ -515,9 +539,9 @@ struct Launcher {
     if (method == M_SS_SYM_RK4) {
       if (o.wdm) return hipErrorNotSupported;
       if (o.gain == GAIN_YB)
-        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, true>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, true, 0, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
       else
-        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, false>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+        hipLaunchKernelGGL((k_fixed<N, T, M_SS_SYM_RK4, NL_GENERIC, false, 0, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                            ws);
       return hipGetLastError();
     }
@@ -532,8 +556,8 @@ struct Launcher {
     else if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_SATURABLE && o.simp_flat)
       hipLaunchKernelGGL((k_fixed<N, T, M_RK4IP, NL_ALL, false, 1 + SPG_SAT, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                          ws);
-    else if (method == M_RK4IP) GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
-    else GN_BY_KIND_GAIN(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else if (method == M_RK4IP) GN_BY_KIND_GAIN_C(o, k_fixed<N COMMA T COMMA M_RK4IP, C, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND_GAIN_C(o, k_fixed<N COMMA T COMMA M_SS_SYMMETRIC, C, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t adaptive(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
@@ -541,10 +565,10 @@ struct Launcher {
     if (method == M_SS_SYM_RK4) {
       if (o.wdm) return hipErrorNotSupported;
       if (o.gain == GAIN_YB)
-        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, true>), dim3(batch), dim3(T), 0, s, o, p, in, out,
+        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, true, 0, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                            ws);
       else
-        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, false>), dim3(batch), dim3(T), 0, s, o, p, in,
+        hipLaunchKernelGGL((k_adaptive<N, T, M_SS_SYM_RK4, NL_GENERIC, false, 0, C>), dim3(batch), dim3(T), 0, s, o, p, in,
                            out, ws);
       return hipGetLastError();
     }
@@ -560,8 +584,8 @@ struct Launcher {
       hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, true, 1 + SPG_YB, C>), dim3(batch), dim3(T), 0, s, o, p, in, out,
                          ws);
     else if (method == M_RK4IP)
-      GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_RK4IP, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
-    else GN_BY_KIND_GAIN(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+      GN_BY_KIND_GAIN_C(o, k_adaptive<N COMMA T COMMA M_RK4IP, C, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
+    else GN_BY_KIND_GAIN_C(o, k_adaptive<N COMMA T COMMA M_SS_SYMMETRIC, C, dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     return hipGetLastError();
   }
   static hipError_t characts(hipStream_t s, long long count, const double2* A, const double2* S, const double* w,

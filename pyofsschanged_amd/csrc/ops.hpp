@@ -154,6 +154,15 @@ __device__ __forceinline__ void pw_sp_entry(double2 (&x)[P], const double2* E, b
 #pragma unroll
   for (int k = 0; k < P; ++k) AIs[T * k] = cmul(e[k], x[k]);
 }
+// The same with A~ already in x (registers).
+template <int P, int T>
+__device__ __forceinline__ void pw_sp_entry_reg(const double2 (&x)[P], const double2* E, bool esq, double2* AIs) {
+  double2 e[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) e[k] = efac(E[T * k], esq);
+#pragma unroll
+  for (int k = 0; k < P; ++k) AIs[T * k] = cmul(e[k], x[k]);
+}
 // Stage ST of rk4ip_sp once x holds F+(x R(x)) of the stage's field (or,
 // when !fresh, Nt itself): Nt = M F+(.)/N (stored to nA when given); k~ = h
 // Nt (times E at stage 1) enters the bracket S of A'~; x <- conj of the
@@ -910,9 +919,13 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB, CFG>& d, double2 (&x
 // s = 1..4, the Raman convolution of the stage's field, the forward
 // transform of x R(x) and the inverse transform of the next stage's field;
 // the N-point transform runs at one call site.
-// Input field: time domain in `in`, spectral in `inS`. Result: time domain
+// Input field: time domain in `in`, spectral in `inS` (without a gain, inS
+// may be null when Nt(in) is not stored: the spectrum is then formed from
+// `in` with one more transform). Result: time domain
 // in x and, when outS is given, spectral in outS (neither of them one of
-// the slots S_K0, S_K1, S_U used here). The kernel's LDS holds
+// the slots S_K0, S_K1, S_U used here; `in` may be S_U when Nt(in) is not
+// ready, since it is read before the first field is parked there). The
+// kernel's LDS holds
 // sp_lds_doubles<N, T>() doubles.
 // GK: the gain in every linear call (SpGain): SPG_SAT, the saturable gain
 // of amplifier.py:149-158 (energies of the arrays the reference passes: A
@@ -957,9 +970,14 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB, CFG>& d, double2 (&x)[
     d.load(x, nA);
     p0 = 1;
   } else {  // the product after the convolution reads A from `in`
+    if (!inS) {  // no stored spectrum: A~ = fft(A) formed here, A_I~ = E A~
+      d.load(x, in);
+      d.fft(x);
+      pw_sp_entry_reg<P, T>(x, d.mine(E), esq, d.mine(sAIs));
+    }
     d.load(x, in);
     raman_stage(d, x);
-    pw_sp_entry<P, T>(x, d.mine(E), esq, d.mine(inS), d.mine(sAIs));
+    if (inS) pw_sp_entry<P, T>(x, d.mine(E), esq, d.mine(inS), d.mine(sAIs));
   }
   bool skip = nA_ready;  // the next pass has no transform (Nt(A) was loaded)
 #pragma unroll 1
