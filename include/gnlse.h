@@ -63,7 +63,8 @@ enum {
   GNLSE_STATUS_SUITABLE_STEP = 2,   /* SuitableStepSizeError       stepper.py:353-354 */
   GNLSE_STATUS_MAX_STEPS = 3,       /* MaximumStepsAllocatedError  stepper.py:376 */
   GNLSE_STATUS_TRACE_OVERFLOW = 4,  /* trace buffer too small (engine-only) */
-  GNLSE_STATUS_REPLAY_EXHAUSTED = 5 /* replay list shorter than the run (engine-only) */
+  GNLSE_STATUS_REPLAY_EXHAUSTED = 5, /* replay list shorter than the run (engine-only) */
+  GNLSE_STATUS_PAIR_TIMEOUT = 6     /* two-workgroup schedule: partner never arrived (engine-only) */
 };
 
 /* Adaptive step-doubling controller constants, stepper.py:133-139, 256-274 */

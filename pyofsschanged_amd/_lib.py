@@ -34,6 +34,7 @@ STATUS_SUITABLE_STEP = 2
 STATUS_MAX_STEPS = 3
 STATUS_TRACE_OVERFLOW = 4
 STATUS_REPLAY_EXHAUSTED = 5
+STATUS_PAIR_TIMEOUT = 6
 
 
 class EngineUnavailableError(RuntimeError):

@@ -31,8 +31,9 @@ const LaunchTable* find_table(long long n) {
 
 // Alternative schedules of the spectral-domain adaptive RK4IP kernels
 // (sched_<N>.hip); schedule 0 of a grid is its launch table's own.
-#define GN_SCHEDULES(X) \
-  X(4096, 1) X(4096, 2) X(4096, 3) X(8192, 1) X(8192, 2) X(8192, 3) X(8192, 4) X(16384, 1) X(16384, 2)
+#define GN_SCHEDULES(X)                                                                                   \
+  X(4096, 1) X(4096, 2) X(4096, 3) X(4096, 4) X(8192, 1) X(8192, 2) X(8192, 3) X(8192, 4) X(8192, 5) X(16384, 1) \
+  X(16384, 2) X(16384, 3)
 #define GN_DECLARE_SCHEDULE(NN, ID) const Schedule* schedule_##NN##_##ID();
 GN_SCHEDULES(GN_DECLARE_SCHEDULE)
 #undef GN_DECLARE_SCHEDULE
@@ -53,7 +54,7 @@ static std::vector<const Schedule*> grid_schedules(long long n) {
 
 // "T512 N:32L.16.16 R:8.8.8.8 tile4": threads, the pass radices of the
 // N-point and N/2-point transforms (L: lane-pair pass) and the stage block.
-static std::string schedule_desc(long long n, int t, int cfg) {
+static std::string schedule_desc(long long n, int t, int cfg, int pair = 0) {
   const int renum = ((cfg & 8) || (cfg & 0x80)) ? 16 : 0;
   auto plan = [&](long long len, int pl) {
     const int p = (int)(len / t);
@@ -68,8 +69,8 @@ static std::string schedule_desc(long long n, int t, int cfg) {
   };
   const int tb = (cfg >> 8) & 3;
   const int tile = tb == 1 ? 2 : tb == 2 ? 8 : 4;
-  return "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) + " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) +
-         " tile" + std::to_string(tile);
+  return std::string(pair ? "pair " : "") + "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) +
+         " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) + " tile" + std::to_string(tile);
 }
 }  // namespace gn
 
@@ -232,7 +233,7 @@ int gnlse_ctx_destroy(gnlse_ctx* c) {
                     &c->ws,   &c->io_in, &c->io_out, &c->io_b, &c->tr_A, &c->tr_z,  &c->st_z,  &c->st_h,
                     &c->lg_h, &c->lg_d,  &c->lg_a,  &c->rp_h, &c->rp_a,  &c->res,   &c->ntr,
                     &c->ch_w, &c->ch_out, &c->ch_npk, &c->ch_pk, &c->yb_rs, &c->yb_ip, &c->yb_as, &c->yb_es,
-                    &c->yb_pp, &c->lin2, &c->efix2, &c->rows};
+                    &c->yb_pp, &c->lin2, &c->efix2, &c->rows, &c->pair_flags};
   for (DevBuf* b : bufs) release(*b);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -569,20 +570,40 @@ static int adaptive_params(int method, const gnlse_adaptive_params* p, AdaptiveP
   return GNLSE_OK;
 }
 
-// The adaptive launch through the context's schedule (the grid's table for
+// The context's schedule for this launch (null: the grid's table, for
 // schedule 0 and for launches no schedule covers).
-static hipError_t adaptive_enqueue(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm,
-                                   const double2* in, double2* out, double2* ws) {
+static const Schedule* active_schedule(gnlse_ctx* c, int method) {
   if (c->sched > 0 && schedule_applies(c->ops, method)) {
     const std::vector<const Schedule*> v = grid_schedules(c->n);
-    if (c->sched <= (int)v.size()) return v[c->sched - 1]->adaptive(c->stream, c->ops, prm, batch, in, out, ws);
+    if (c->sched <= (int)v.size()) return v[c->sched - 1];
   }
+  return nullptr;
+}
+
+// The adaptive launch through a single-workgroup schedule or the grid's
+// table; no allocation (the no-allocation entry point uses it).
+static hipError_t adaptive_enqueue(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm,
+                                   const double2* in, double2* out, double2* ws) {
+  const Schedule* sc = active_schedule(c, method);
+  if (sc && !sc->pair) return sc->adaptive(c->stream, c->ops, prm, batch, in, out, ws, nullptr);
   return c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, ws);
 }
 
 static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm, const double2* in,
                            double2* out) {
   RET_IF(check_pump(c, batch));
+  const Schedule* sc = active_schedule(c, method);
+  if (sc && sc->pair) {  // two workgroups per instance: twice the workspace, two zeroed flags per instance
+    RET_IF(ensure_ws(c, 2 * batch));
+    RET_IF(ensure(c->pair_flags, (size_t)2 * batch * sizeof(unsigned)));
+    HIPCHK(hipMemsetAsync(c->pair_flags.p, 0, (size_t)2 * batch * sizeof(unsigned), c->stream));
+    RET_IF(timed_begin(c));
+    const hipError_t e =
+        sc->adaptive(c->stream, c->ops, prm, batch, in, out, (double2*)c->ws.p, (unsigned*)c->pair_flags.p);
+    if (e != hipErrorCooperativeLaunchTooLarge) return timed_end(c, e);
+    (void)hipGetLastError();  // too many instances to keep both workgroups resident: the grid's default
+    return timed_end(c, c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p));
+  }
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
   return timed_end(c, adaptive_enqueue(c, method, batch, prm, in, out, (double2*)c->ws.p));
@@ -785,10 +806,11 @@ int gnlse_schedule_info(int64_t n, int index, int* threads, int* cfg, char* desc
   if (index < 0 || index > (int)v.size()) return fail(GNLSE_ERR_ARG, "schedule index out of range");
   const int th = index == 0 ? t->t : v[index - 1]->t;
   const int cf = index == 0 ? t->cfg : v[index - 1]->cfg;
+  const int pr = index == 0 ? 0 : v[index - 1]->pair;
   if (threads) *threads = th;
   if (cfg) *cfg = cf;
   if (desc && desc_len > 0) {
-    const std::string d = t->blocks > 1 ? "T" + std::to_string(th) + " two-level" : schedule_desc(n, th, cf);
+    const std::string d = t->blocks > 1 ? "T" + std::to_string(th) + " two-level" : schedule_desc(n, th, cf, pr);
     std::strncpy(desc, d.c_str(), (size_t)desc_len - 1);
     desc[desc_len - 1] = 0;
   }

@@ -78,10 +78,14 @@ const LaunchTable* find_table(long long n);
 // gain): threads per instance and Dev's CFG (transform plans, stage-pass
 // block). A grid's default schedule is its LaunchTable's; the autotuner
 // (gnlse_tune_adaptive) times the alternatives on the caller's inputs.
+// pair = 1: two workgroups per instance (k_adaptive_pair): the launch needs
+// twice the workspace, two flags per instance set to zero and room for every
+// workgroup at once (hipErrorCooperativeLaunchTooLarge otherwise, and the
+// caller falls back to the grid's default).
 struct Schedule {
-  int t, cfg;
+  int t, cfg, pair;
   hipError_t (*adaptive)(hipStream_t, const Ops&, const AdaptiveParams&, long long batch, const double2* in,
-                         double2* out, double2* ws);
+                         double2* out, double2* ws, unsigned* flags);
 };
 // The launches a schedule covers: rk4ip with the full nonlinear operator and
 // no gain (SP 1) or the saturable gain with the flat interior Simpson weight
@@ -244,15 +248,53 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // (read only by the first call after an acceptance, before that call parks
 // its first stage field there) and the fine step forms the half step's
 // spectrum from its time samples instead of reading it from S_HS.
-template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0, int CFG = 0>
+// Two workgroups per instance (PAIR): the half and fine steps (role 0) and
+// the coarse step (role 1) of an attempt are independent up to the error
+// norm, so they run side by side and an attempt waits for two rk4ip calls
+// instead of three. Each role has its own workspace (slots of instance
+// 2 b + role) and computes N(A) for its own step; the fine and coarse
+// results go to slot S_AI or S_EH of their role (by attempt parity, so an
+// attempt never overwrites what the partner may still read), the roles meet
+// at pair_sync, and both then form the same error norm, controller decision
+// and accepted field from both results (the same operations in the same
+// order: identical values in both roles and equal, bit for bit, to the
+// single-workgroup kernel). Role 0 alone writes traces, the attempt log, the
+// output and the result. The launch must keep both workgroups of a pair
+// resident (a cooperative launch); the wait is bounded
+// (GNLSE_STATUS_PAIR_TIMEOUT).
+constexpr long long kPairSpin = 1ll << 26;  // s_sleep(10) spins, about 20 s
+__device__ __forceinline__ bool pair_sync(unsigned* flags, int role, unsigned epoch) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this role's results visible to the partner
+    __hip_atomic_store(&flags[role], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long long it = 0;
+    while (__hip_atomic_load(&flags[1 - role], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch && it < kPairSpin) {
+      __builtin_amdgcn_s_sleep(10);
+      ++it;
+    }
+    ok = it < kPairSpin;
+    __threadfence();  // the partner's results visible here
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+template <int N, int T, int METHOD, int K, bool YB = false, int SP = 0, int CFG = 0, bool PAIR = false>
 __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Ops& ops, const AdaptiveParams& prm,
                                               size_t b, const double2* __restrict__ in, double2* __restrict__ out,
-                                              double2* ws_all, double2* __restrict__ spec) {
+                                              double2* ws_all, double2* __restrict__ spec, int role = 0,
+                                              unsigned* pflags = nullptr) {
   static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
                 "spectral-domain stepping: rk4ip, NL_ALL");
+  static_assert(!PAIR || (SP == 1 || SP == 2), "two-workgroup instances: spectral-domain rk4ip without the Yb gain");
   Dev<N, T, K, YB, CFG> d(lds, red, ops);
   constexpr int P = N / T;
-  double2* ws = ws_all + b * NSLOT * N;
+  double2* ws = ws_all + (PAIR ? 2 * b + role : b) * NSLOT * N;
+  double2* ws_f = ws_all + (2 * b) * NSLOT * N;      // PAIR: role 0 (fine result)
+  double2* ws_c = ws_all + (2 * b + 1) * NSLOT * N;  // PAIR: role 1 (coarse result)
+  const bool lead = !PAIR || role == 0;
   double2* sA = ws + S_A * N;
   double2* sH = ws + S_H * N;
   double2* sEQ = ws + S_EQ * N;
@@ -272,13 +314,15 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   int status = GNLSE_STATUS_OK;
   bool finished = false;
 
-  auto store_trace = [&](double zt, double ht) {
+  auto store_trace = [&](double zt, double ht) {  // both roles count, role 0 stores
     if (n_tr < prm.max_traces) {
-      d.store(prm.tr_A + (b * prm.max_traces + n_tr) * N, x);
-      if (threadIdx.x == 0) {
-        prm.tr_z[b * prm.max_traces + n_tr] = zt;
-        prm.st_z[b * prm.max_traces + n_tr] = zt;
-        prm.st_h[b * prm.max_traces + n_tr] = ht;
+      if (lead) {
+        d.store(prm.tr_A + (b * prm.max_traces + n_tr) * N, x);
+        if (threadIdx.x == 0) {
+          prm.tr_z[b * prm.max_traces + n_tr] = zt;
+          prm.st_z[b * prm.max_traces + n_tr] = zt;
+          prm.st_h[b * prm.max_traces + n_tr] = ht;
+        }
       }
       ++n_tr;
       return true;
@@ -313,8 +357,11 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       // stepper.py:303-308 in its order (the Yb pump state depends on it):
       // half step, fine step (both h/2), then the coarse step h. The fine
       // result replaces the half-step field in sH; the coarse one stays in x.
+      // PAIR: the result slot of this attempt in each role's workspace
+      const int rs = (n_att & 1) ? S_EH : S_AI;
 #pragma unroll 1
       for (int c = 0; c < 3; ++c) {
+        if (PAIR && ((c == 2) != (role == 1))) continue;
         const double2* src = (c == 1) ? sH : sAt;
         const double2* E = (c == 2) ? sEH : sEQ;
         // N(A) is shared by the half and the coarse step, and by every
@@ -326,13 +373,23 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         else
           integrator<METHOD>(d, x, src, E, c == 2 ? h : h_half, ws, sNA, from_a && nA_valid, from_a && !nA_valid);
         if (from_a) nA_valid = true;
-        if (c < 2) d.store(sH, x);
+        if (PAIR && c >= 1) d.store(ws + rs * N, x);  // fine (role 0) or coarse (role 1) result
+        else if (c < 2) d.store(sH, x);
+      }
+      const double2* sF = sH;  // the fine result; x holds the coarse one
+      if constexpr (PAIR) {
+        if (!pair_sync(pflags, role, (unsigned)(n_att + 1))) {
+          status = GNLSE_STATUS_PAIR_TIMEOUT;
+          break;
+        }
+        sF = ws_f + rs * N;
+        d.load(x, ws_c + rs * N);
       }
       // stepper.py:230-245 relative_local_error
       double sd = 0.0, sf = 0.0, bad = 0.0;
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        const double2 f = sH[d.idx(k)];
+        const double2 f = sF[d.idx(k)];
         const double2 df = csub(f, x[k]);
         sd += abs2(df);
         sf += abs2(f);
@@ -360,7 +417,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         h = h_temp * prm.max_factor;
       }
       const bool acc = prm.replay_h ? (prm.replay_accept[n_att] != 0) : (delta < 2.0 * prm.tol);
-      if (threadIdx.x == 0 && n_att < prm.log_cap) {
+      if (lead && threadIdx.x == 0 && n_att < prm.log_cap) {
         prm.log_h[b * prm.log_cap + n_att] = h_temp;
         prm.log_delta[b * prm.log_cap + n_att] = delta;
         prm.log_acc[b * prm.log_cap + n_att] = acc ? 1 : 0;
@@ -373,7 +430,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
         if constexpr (!(SP >= 2 || K == NL_KERR)) {
 #pragma unroll
           for (int k = 0; k < P; ++k)
-            x[k] = csub(cscale(sH[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
+            x[k] = csub(cscale(sF[d.idx(k)], prm.f_alpha), cscale(x[k], prm.f_beta));
         }
         d.store(sAt, x);
         nA_valid = false;
@@ -404,13 +461,14 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
   }
   if (!finished && status == GNLSE_STATUS_OK) status = GNLSE_STATUS_MAX_STEPS;
   d.pump_end(b);
+  if (!lead && status != GNLSE_STATUS_PAIR_TIMEOUT) return;
   if (SP == 1 && !finished) {  // S_U may hold a stage field: the accepted field from its spectrum
     d.load(x, sAs);
     d.ifft(x);
   } else {
     d.load(x, sAt);
   }
-  d.store(out + b * N, x);
+  if (lead) d.store(out + b * N, x);
   if (spec) {
     d.fft(x);
     d.store(spec + b * N, x);
@@ -434,6 +492,26 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
   __shared__ double lds[SP ? sp_lds_doubles<N, T, CFG>() : Dev<N, T, K, YB, CFG>::F::LDS_DOUBLES];
   __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
   adaptive_body<N, T, METHOD, K, YB, SP, CFG>(lds, red, ops, prm, blockIdx.x, in, out, ws_all, nullptr);
+}
+
+// Two workgroups per instance (see pair_sync): workgroup w serves instance
+// 8 (w / 16) + w % 8 in role (w / 8) % 2, so both roles of an instance sit
+// in the same position of two consecutive groups of eight (the same XCD
+// when workgroups are dealt to the eight XCDs in turn). Workgroups past the
+// batch return at once (they are nobody's partner). flags: two epoch
+// counters per instance, zero at launch.
+template <int N, int T, int SP, int C>
+__global__ void __launch_bounds__(T) k_adaptive_pair(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
+                                                     double2* __restrict__ out, double2* ws_all, unsigned* flags,
+                                                     long long batch) {
+  __shared__ double lds[sp_lds_doubles<N, T, C>()];
+  __shared__ double red[18 * (T / 64) + 2];
+  const long long w = blockIdx.x;
+  const long long b = (w / 16) * 8 + (w % 8);
+  const int role = (int)((w / 8) % 2);
+  if (b >= batch) return;
+  adaptive_body<N, T, M_RK4IP, NL_ALL, false, SP, C, true>(lds, red, ops, prm, b, in, out, ws_all, nullptr, role,
+                                                          flags + 2 * b);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
@@ -600,7 +678,7 @@ struct Launcher {
 template <int N, int T, int C>
 struct SchedLauncher {
   static hipError_t adaptive(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
-                             double2* out, double2* ws) {
+                             double2* out, double2* ws, unsigned*) {
     if (o.gain == GAIN_NONE)
       hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.gain == GAIN_SATURABLE && o.simp_flat)
@@ -608,7 +686,34 @@ struct SchedLauncher {
     else return hipErrorNotSupported;
     return hipGetLastError();
   }
-  static constexpr Schedule sched = {T, C, &adaptive};
+  static constexpr Schedule sched = {T, C, 0, &adaptive};
+};
+
+template <int N, int T, int C>
+struct PairLauncher {
+  template <int SP>
+  static hipError_t go(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
+                       double2* out, double2* ws, unsigned* flags) {
+    const void* k = reinterpret_cast<const void*>(&k_adaptive_pair<N, T, SP, C>);
+    const long long grid = 16 * ((batch + 7) / 8);
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, T, 0);
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if ((long long)per_cu * cus < grid) return hipErrorCooperativeLaunchTooLarge;
+    Ops ops = o;
+    AdaptiveParams prm = p;
+    void* args[] = {&ops, &prm, &in, &out, &ws, &flags, &batch};
+    return hipLaunchCooperativeKernel(k, dim3((unsigned)grid), dim3(T), args, 0, s);
+  }
+  static hipError_t adaptive(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
+                             double2* out, double2* ws, unsigned* flags) {
+    if (o.gain == GAIN_NONE) return go<1>(s, o, p, batch, in, out, ws, flags);
+    if (o.gain == GAIN_SATURABLE && o.simp_flat) return go<2>(s, o, p, batch, in, out, ws, flags);
+    return hipErrorNotSupported;
+  }
+  static constexpr Schedule sched = {T, C, 1, &adaptive};
 };
 
 }  // namespace gn
@@ -616,6 +721,10 @@ struct SchedLauncher {
 #define GN_DEFINE_SCHEDULE(NN, TT, CC, ID) \
   namespace gn {                            \
   const Schedule* schedule_##NN##_##ID() { return &SchedLauncher<NN, TT, CC>::sched; } \
+  }
+#define GN_DEFINE_PAIR_SCHEDULE(NN, TT, CC, ID) \
+  namespace gn {                                 \
+  const Schedule* schedule_##NN##_##ID() { return &PairLauncher<NN, TT, CC>::sched; } \
   }
 
 #define GN_DEFINE_GRID(NN, TT, CC) \

@@ -135,3 +135,54 @@ def test_schedule_selection_errors(lib):
     ctx.set_schedule(1)
     ctx.set_grid(8192)
     assert ctx.get_schedule() == 0
+
+
+def _pair_index(lib, N):
+    return next(s["index"] for s in lib.schedules(N) if s["desc"].startswith("pair "))
+
+
+def test_pair_schedule_equals_default_bit_for_bit(lib):
+    """Two workgroups per instance (half and fine steps beside the coarse
+    step) compute the same operations in the same order as one workgroup:
+    outputs, attempt logs and results are equal bit for bit, with traces."""
+    N = 16384
+    ctx, fib, A, refs, logs = amplifier_batch_ctx(lib, N)
+    params = adaptive_params(lib, fib, 1e-6, traces=4)
+    cap = max(len(l) for l in logs) + 8
+    ctx.set_schedule(0)
+    one = ctx.propagate_adaptive(0, A, params, max_traces=8, log_cap=cap)
+    ctx.set_schedule(_pair_index(lib, N))
+    two = ctx.propagate_adaptive(0, A, params, max_traces=8, log_cap=cap)
+    print(f"pair vs default at {N}: kernel {ctx.last_kernel_ms():.1f} ms", flush=True)
+    assert np.array_equal(one["A"], two["A"])
+    for key in ("log_h", "log_delta", "log_accept", "traces", "trace_z"):
+        if key in one:
+            assert np.array_equal(np.asarray(one[key]), np.asarray(two[key])), key
+    for r1, r2 in zip(one["results"], two["results"]):
+        assert r1 == r2
+    for i, ref in enumerate(refs):
+        assert two["results"][i]["n_attempts"] == len(logs[i])
+        assert rel_l2(two["A"][i], ref) <= FIELD_TOL
+
+
+def test_pair_schedule_falls_back_when_the_batch_does_not_fit(lib):
+    """A batch whose workgroups cannot all be resident (two per instance)
+    runs the grid's default kernel: the same results as schedule 0."""
+    N = 8192
+    dom, A, fib, ref, log = c2_reference(N)
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
+    ctx = engine_context(dom, lin, nl, 1)
+    fib = dict(fib, length=2e-6)
+    params = adaptive_params(lib, fib, 1e-6)
+    batch = np.stack([A * (1 + 1e-4 * k) for k in range(300)])
+    ctx.set_schedule(0)
+    one = ctx.propagate_adaptive(0, batch, params)
+    ctx.set_schedule(_pair_index(lib, N))
+    two = ctx.propagate_adaptive(0, batch, params)
+    assert np.array_equal(one["A"], two["A"])
+    assert all(r["status"] == 0 for r in two["results"])
+
+
+def amplifier_batch_ctx(lib, N):
+    from test_gpu_propagation import amplifier_batch
+    return amplifier_batch(lib, N, 2e-4, (-2.0, 0.5, 2.0))
