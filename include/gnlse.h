@@ -287,6 +287,17 @@ int gnlse_schedule_info(int64_t n, int index, int* threads, int* cfg, char* desc
 /* Select the schedule of the context's launches (0 = default) / read it. */
 int gnlse_set_schedule(gnlse_ctx* ctx, int index);
 int gnlse_get_schedule(gnlse_ctx* ctx, int* index);
+/* Two workgroups per instance (engine-only). With the default schedule and
+ * pairing on (the default), an adaptive propagation whose batch leaves room
+ * on the device (every workgroup resident at once, e.g. a single pyofss
+ * Fibre run, or 64 instances on 256 CUs) runs each instance on two
+ * workgroups: the coarse step beside the half and fine steps of every
+ * attempt, one meeting per attempt; results are equal bit for bit to one
+ * workgroup per instance. Grids of 4096-16384 samples; rk4ip with the full
+ * Raman + self-steepening operator (no gain or the saturable gain), rk4ip
+ * and ss_symmetric with the Kerr operator; single-channel fields.
+ * enable = 0 keeps one workgroup per instance. */
+int gnlse_set_pairing(gnlse_ctx* ctx, int enable);
 /* Autotuner: times every schedule of the context's grid on trial_attempts
  * replayed steps of p->h0 (all accepted) of `batch` instances of the host
  * inputs A_in, `repeats` runs each (the first one untimed), selects the

@@ -118,6 +118,7 @@ SIGNATURES = {
                             _INT],
     "gnlse_set_schedule": [_P, _INT],
     "gnlse_get_schedule": [_P, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_set_pairing": [_P, _INT],
     "gnlse_tune_adaptive": [_P, _I64, ctypes.POINTER(AdaptiveParams), _P, _I64, _INT, ctypes.POINTER(ctypes.c_int),
                             _P],
     "gnlse_trace_characteristics": [_P, _I64, _P, _P, _P, _P, _P, _INT],
@@ -475,6 +476,11 @@ class Context:
         """Select the schedule of the spectral-domain RK4IP launches (0 is
         the grid's default; see schedules())."""
         self._call("gnlse_set_schedule", int(index))
+
+    def set_pairing(self, enable):
+        """Two workgroups per instance for adaptive runs whose batch leaves
+        room on the device (on by default; results are identical)."""
+        self._call("gnlse_set_pairing", 1 if enable else 0)
 
     def get_schedule(self):
         v = ctypes.c_int()

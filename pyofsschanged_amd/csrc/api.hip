@@ -31,9 +31,8 @@ const LaunchTable* find_table(long long n) {
 
 // Alternative schedules of the spectral-domain adaptive RK4IP kernels
 // (sched_<N>.hip); schedule 0 of a grid is its launch table's own.
-#define GN_SCHEDULES(X)                                                                                   \
-  X(4096, 1) X(4096, 2) X(4096, 3) X(4096, 4) X(8192, 1) X(8192, 2) X(8192, 3) X(8192, 4) X(8192, 5) X(16384, 1) \
-  X(16384, 2) X(16384, 3)
+#define GN_SCHEDULES(X) \
+  X(4096, 1) X(4096, 2) X(4096, 3) X(8192, 1) X(8192, 2) X(8192, 3) X(8192, 4) X(16384, 1) X(16384, 2)
 #define GN_DECLARE_SCHEDULE(NN, ID) const Schedule* schedule_##NN##_##ID();
 GN_SCHEDULES(GN_DECLARE_SCHEDULE)
 #undef GN_DECLARE_SCHEDULE
@@ -54,7 +53,7 @@ static std::vector<const Schedule*> grid_schedules(long long n) {
 
 // "T512 N:32L.16.16 R:8.8.8.8 tile4": threads, the pass radices of the
 // N-point and N/2-point transforms (L: lane-pair pass) and the stage block.
-static std::string schedule_desc(long long n, int t, int cfg, int pair = 0) {
+static std::string schedule_desc(long long n, int t, int cfg) {
   const int renum = ((cfg & 8) || (cfg & 0x80)) ? 16 : 0;
   auto plan = [&](long long len, int pl) {
     const int p = (int)(len / t);
@@ -69,8 +68,8 @@ static std::string schedule_desc(long long n, int t, int cfg, int pair = 0) {
   };
   const int tb = (cfg >> 8) & 3;
   const int tile = tb == 1 ? 2 : tb == 2 ? 8 : 4;
-  return std::string(pair ? "pair " : "") + "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) +
-         " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) + " tile" + std::to_string(tile);
+  return "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) + " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) +
+         " tile" + std::to_string(tile);
 }
 }  // namespace gn
 
@@ -580,29 +579,29 @@ static const Schedule* active_schedule(gnlse_ctx* c, int method) {
   return nullptr;
 }
 
-// The adaptive launch through a single-workgroup schedule or the grid's
-// table; no allocation (the no-allocation entry point uses it).
+// The one-workgroup adaptive launch through the context's schedule or the
+// grid's table; no allocation (the no-allocation entry point uses it).
 static hipError_t adaptive_enqueue(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm,
                                    const double2* in, double2* out, double2* ws) {
   const Schedule* sc = active_schedule(c, method);
-  if (sc && !sc->pair) return sc->adaptive(c->stream, c->ops, prm, batch, in, out, ws, nullptr);
+  if (sc) return sc->adaptive(c->stream, c->ops, prm, batch, in, out, ws);
   return c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, ws);
 }
 
 static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const AdaptiveParams& prm, const double2* in,
                            double2* out) {
   RET_IF(check_pump(c, batch));
-  const Schedule* sc = active_schedule(c, method);
-  if (sc && sc->pair) {  // two workgroups per instance: twice the workspace, two zeroed flags per instance
-    RET_IF(ensure_ws(c, 2 * batch));
+  // Two workgroups per instance (the grid's default schedule, pairing on)
+  // when the operators have that kernel and every workgroup of the batch
+  // fits on the device at once; else one workgroup per instance.
+  if (c->pairing && c->sched == 0 && c->tab->adaptive_pair && !c->ops.wdm) {
+    RET_IF(ensure(c->ws, (size_t)2 * batch * kPairSlots * c->n * sizeof(double2)));
     RET_IF(ensure(c->pair_flags, (size_t)2 * batch * sizeof(unsigned)));
     HIPCHK(hipMemsetAsync(c->pair_flags.p, 0, (size_t)2 * batch * sizeof(unsigned), c->stream));
     RET_IF(timed_begin(c));
-    const hipError_t e =
-        sc->adaptive(c->stream, c->ops, prm, batch, in, out, (double2*)c->ws.p, (unsigned*)c->pair_flags.p);
-    if (e != hipErrorCooperativeLaunchTooLarge) return timed_end(c, e);
-    (void)hipGetLastError();  // too many instances to keep both workgroups resident: the grid's default
-    return timed_end(c, c->tab->adaptive(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p));
+    const hipError_t e = c->tab->adaptive_pair(c->stream, c->ops, method, prm, batch, in, out, (double2*)c->ws.p,
+                                               (unsigned*)c->pair_flags.p);
+    if (e != hipErrorNotSupported && e != hipErrorCooperativeLaunchTooLarge) return timed_end(c, e);
   }
   RET_IF(ensure_ws(c, batch));
   RET_IF(timed_begin(c));
@@ -806,11 +805,10 @@ int gnlse_schedule_info(int64_t n, int index, int* threads, int* cfg, char* desc
   if (index < 0 || index > (int)v.size()) return fail(GNLSE_ERR_ARG, "schedule index out of range");
   const int th = index == 0 ? t->t : v[index - 1]->t;
   const int cf = index == 0 ? t->cfg : v[index - 1]->cfg;
-  const int pr = index == 0 ? 0 : v[index - 1]->pair;
   if (threads) *threads = th;
   if (cfg) *cfg = cf;
   if (desc && desc_len > 0) {
-    const std::string d = t->blocks > 1 ? "T" + std::to_string(th) + " two-level" : schedule_desc(n, th, cf, pr);
+    const std::string d = t->blocks > 1 ? "T" + std::to_string(th) + " two-level" : schedule_desc(n, th, cf);
     std::strncpy(desc, d.c_str(), (size_t)desc_len - 1);
     desc[desc_len - 1] = 0;
   }
@@ -821,6 +819,12 @@ int gnlse_set_schedule(gnlse_ctx* c, int index) {
   RET_IF(check_ready(c, false, false));
   if (index < 0 || index > (int)grid_schedules(c->n).size()) return fail(GNLSE_ERR_ARG, "schedule index out of range");
   c->sched = index;
+  return GNLSE_OK;
+}
+
+int gnlse_set_pairing(gnlse_ctx* c, int enable) {
+  if (!c) return fail(GNLSE_ERR_ARG, "null context");
+  c->pairing = enable ? 1 : 0;
   return GNLSE_OK;
 }
 

@@ -61,7 +61,8 @@ struct gnlse_ctx {
   DevBuf rows;                         // row index table of gnlse_copy_rows
   void* scratch = nullptr;             // this device's no-allocation workspace (device address)
   int sched = 0;                       // schedule of the spectral-domain RK4IP launches (0: the grid's default)
-  DevBuf pair_flags;                   // two-workgroup schedules: two epoch counters per instance
+  DevBuf pair_flags;                   // two-workgroup launches: two epoch counters per instance
+  int pairing = 1;                     // two workgroups per instance when the batch leaves room
 };
 
 // Fields per instance: 2 with the WDM channels, else 1.

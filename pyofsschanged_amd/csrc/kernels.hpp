@@ -69,6 +69,13 @@ struct LaunchTable {
                          double* out, int* npk, int* pk, int max_pk, double2* scratch);
   // schedule (Dev's CFG) of the spectral-domain RK4IP kernels in this table
   int cfg;
+  // adaptive propagation on two workgroups per instance (k_adaptive_pair;
+  // workspace 2 kPairSlots N per instance, two zeroed flags per instance):
+  // hipErrorNotSupported for operators without a two-workgroup kernel,
+  // hipErrorCooperativeLaunchTooLarge when the batch does not fit; null on
+  // grids without one
+  hipError_t (*adaptive_pair)(hipStream_t, const Ops&, int method, const AdaptiveParams&, long long batch,
+                              const double2* in, double2* out, double2* ws, unsigned* flags);
 };
 
 const LaunchTable* find_table(long long n);
@@ -78,14 +85,10 @@ const LaunchTable* find_table(long long n);
 // gain): threads per instance and Dev's CFG (transform plans, stage-pass
 // block). A grid's default schedule is its LaunchTable's; the autotuner
 // (gnlse_tune_adaptive) times the alternatives on the caller's inputs.
-// pair = 1: two workgroups per instance (k_adaptive_pair): the launch needs
-// twice the workspace, two flags per instance set to zero and room for every
-// workgroup at once (hipErrorCooperativeLaunchTooLarge otherwise, and the
-// caller falls back to the grid's default).
 struct Schedule {
-  int t, cfg, pair;
+  int t, cfg;
   hipError_t (*adaptive)(hipStream_t, const Ops&, const AdaptiveParams&, long long batch, const double2* in,
-                         double2* out, double2* ws, unsigned* flags);
+                         double2* out, double2* ws);
 };
 // The launches a schedule covers: rk4ip with the full nonlinear operator and
 // no gain (SP 1) or the saturable gain with the flat interior Simpson weight
@@ -250,11 +253,12 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // spectrum from its time samples instead of reading it from S_HS.
 // Two workgroups per instance (PAIR): the half and fine steps (role 0) and
 // the coarse step (role 1) of an attempt are independent up to the error
-// norm, so they run side by side and an attempt waits for two rk4ip calls
-// instead of three. Each role has its own workspace (slots of instance
-// 2 b + role) and computes N(A) for its own step; the fine and coarse
-// results go to slot S_AI or S_EH of their role (by attempt parity, so an
-// attempt never overwrites what the partner may still read), the roles meet
+// norm, so they run side by side and an attempt waits for two integrator
+// calls instead of three. Each role has its own workspace of kPairSlots
+// slots (the NSLOT of a single workgroup and two result slots) and computes
+// N(A) for its own step; the fine and coarse results go to result slot
+// n_att mod 2 of their role (so an attempt never overwrites what the
+// partner may still read), the roles meet
 // at pair_sync, and both then form the same error norm, controller decision
 // and accepted field from both results (the same operations in the same
 // order: identical values in both roles and equal, bit for bit, to the
@@ -263,6 +267,7 @@ __global__ void __launch_bounds__(T) k_fixed(Ops ops, FixedParams prm, const dou
 // resident (a cooperative launch); the wait is bounded
 // (GNLSE_STATUS_PAIR_TIMEOUT).
 constexpr long long kPairSpin = 1ll << 26;  // s_sleep(10) spins, about 20 s
+constexpr int kPairSlots = NSLOT + 2;
 __device__ __forceinline__ bool pair_sync(unsigned* flags, int role, unsigned epoch) {
   __shared__ int ok;
   __syncthreads();
@@ -288,12 +293,12 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
                                               unsigned* pflags = nullptr) {
   static_assert(!SP || (METHOD == M_RK4IP && K == NL_ALL && YB == (SP == 1 + SPG_YB)),
                 "spectral-domain stepping: rk4ip, NL_ALL");
-  static_assert(!PAIR || (SP == 1 || SP == 2), "two-workgroup instances: spectral-domain rk4ip without the Yb gain");
+  static_assert(!PAIR || !YB, "two-workgroup instances: not with the Yb gain (its pump follows the call order)");
   Dev<N, T, K, YB, CFG> d(lds, red, ops);
   constexpr int P = N / T;
-  double2* ws = ws_all + (PAIR ? 2 * b + role : b) * NSLOT * N;
-  double2* ws_f = ws_all + (2 * b) * NSLOT * N;      // PAIR: role 0 (fine result)
-  double2* ws_c = ws_all + (2 * b + 1) * NSLOT * N;  // PAIR: role 1 (coarse result)
+  double2* ws = ws_all + (PAIR ? (2 * b + role) * kPairSlots : b * NSLOT) * N;
+  double2* ws_f = ws_all + (2 * b) * kPairSlots * N;      // PAIR: role 0 (fine result)
+  double2* ws_c = ws_all + (2 * b + 1) * kPairSlots * N;  // PAIR: role 1 (coarse result)
   const bool lead = !PAIR || role == 0;
   double2* sA = ws + S_A * N;
   double2* sH = ws + S_H * N;
@@ -358,7 +363,7 @@ __device__ __forceinline__ void adaptive_body(double* lds, double* red, const Op
       // half step, fine step (both h/2), then the coarse step h. The fine
       // result replaces the half-step field in sH; the coarse one stays in x.
       // PAIR: the result slot of this attempt in each role's workspace
-      const int rs = (n_att & 1) ? S_EH : S_AI;
+      const int rs = NSLOT + (int)(n_att & 1);
 #pragma unroll 1
       for (int c = 0; c < 3; ++c) {
         if (PAIR && ((c == 2) != (role == 1))) continue;
@@ -500,18 +505,38 @@ __global__ void __launch_bounds__(T) k_adaptive(Ops ops, AdaptiveParams prm, con
 // when workgroups are dealt to the eight XCDs in turn). Workgroups past the
 // batch return at once (they are nobody's partner). flags: two epoch
 // counters per instance, zero at launch.
-template <int N, int T, int SP, int C>
+template <int N, int T, int METHOD, int K, int SP, int C>
 __global__ void __launch_bounds__(T) k_adaptive_pair(Ops ops, AdaptiveParams prm, const double2* __restrict__ in,
                                                      double2* __restrict__ out, double2* ws_all, unsigned* flags,
                                                      long long batch) {
-  __shared__ double lds[sp_lds_doubles<N, T, C>()];
-  __shared__ double red[18 * (T / 64) + 2];
+  __shared__ double lds[SP ? sp_lds_doubles<N, T, C>() : Dev<N, T, K, false, C>::F::LDS_DOUBLES];
+  __shared__ double red[SP ? 18 * (T / 64) + 2 : T / 64 + 2];
   const long long w = blockIdx.x;
   const long long b = (w / 16) * 8 + (w % 8);
   const int role = (int)((w / 8) % 2);
   if (b >= batch) return;
-  adaptive_body<N, T, M_RK4IP, NL_ALL, false, SP, C, true>(lds, red, ops, prm, b, in, out, ws_all, nullptr, role,
-                                                          flags + 2 * b);
+  adaptive_body<N, T, METHOD, K, false, SP, C, true>(lds, red, ops, prm, b, in, out, ws_all, nullptr, role,
+                                                    flags + 2 * b);
+}
+
+// Cooperative launch of the two-workgroup kernel when every workgroup fits
+// at once (hipErrorCooperativeLaunchTooLarge otherwise: the caller runs the
+// one-workgroup kernel).
+template <int N, int T, int METHOD, int K, int SP, int C>
+hipError_t launch_pair(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
+                       double2* out, double2* ws, unsigned* flags) {
+  const void* k = reinterpret_cast<const void*>(&k_adaptive_pair<N, T, METHOD, K, SP, C>);
+  const long long grid = 16 * ((batch + 7) / 8);
+  int per_cu = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, T, 0);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  if ((long long)per_cu * cus < grid) return hipErrorCooperativeLaunchTooLarge;
+  Ops ops = o;
+  AdaptiveParams prm = p;
+  void* args[] = {&ops, &prm, &in, &out, &ws, &flags, &batch};
+  return hipLaunchCooperativeKernel(k, dim3((unsigned)grid), dim3(T), args, 0, s);
 }
 
 // Dispatch on the nonlinearity: Kerr and the full Raman + self-steepening
@@ -671,14 +696,35 @@ struct Launcher {
     hipLaunchKernelGGL((k_characts<N>), dim3((unsigned)count), dim3(kCharT), 0, s, A, S, w, out, npk, pk, max_pk);
     return hipGetLastError();
   }
+  // The two-workgroup kernels exist on the 4096-16384 grids for rk4ip with
+  // the full operator (no gain or the saturable gain) and for rk4ip and
+  // ss_symmetric with the Kerr operator.
+  static hipError_t adaptive_pair(hipStream_t s, const Ops& o, int method, const AdaptiveParams& p, long long batch,
+                                  const double2* in, double2* out, double2* ws, unsigned* flags) {
+    if constexpr (N < 4096) {
+      return hipErrorNotSupported;
+    } else {
+      if (o.wdm || o.gain == GAIN_YB) return hipErrorNotSupported;
+      if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_NONE)
+        return launch_pair<N, T, M_RK4IP, NL_ALL, 1, C>(s, o, p, batch, in, out, ws, flags);
+      if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_SATURABLE && o.simp_flat)
+        return launch_pair<N, T, M_RK4IP, NL_ALL, 2, C>(s, o, p, batch, in, out, ws, flags);
+      if (o.nl_kind == NL_KERR && o.gain == GAIN_NONE) {
+        if (method == M_RK4IP) return launch_pair<N, T, M_RK4IP, NL_KERR, 0, C>(s, o, p, batch, in, out, ws, flags);
+        if (method == M_SS_SYMMETRIC)
+          return launch_pair<N, T, M_SS_SYMMETRIC, NL_KERR, 0, C>(s, o, p, batch, in, out, ws, flags);
+      }
+      return hipErrorNotSupported;
+    }
+  }
   static constexpr LaunchTable table = {N,       T,     NSLOT, 1,           &fft,  &make_exp, &linear,  &linfac,
-                                        &nonlin, &exp_nonlin, &step, &fixed, &adaptive, &characts, C};
+                                        &nonlin, &exp_nonlin, &step, &fixed, &adaptive, &characts, C, &adaptive_pair};
 };
 
 template <int N, int T, int C>
 struct SchedLauncher {
   static hipError_t adaptive(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
-                             double2* out, double2* ws, unsigned*) {
+                             double2* out, double2* ws) {
     if (o.gain == GAIN_NONE)
       hipLaunchKernelGGL((k_adaptive<N, T, M_RK4IP, NL_ALL, false, 1, C>), dim3(batch), dim3(T), 0, s, o, p, in, out, ws);
     else if (o.gain == GAIN_SATURABLE && o.simp_flat)
@@ -686,35 +732,9 @@ struct SchedLauncher {
     else return hipErrorNotSupported;
     return hipGetLastError();
   }
-  static constexpr Schedule sched = {T, C, 0, &adaptive};
+  static constexpr Schedule sched = {T, C, &adaptive};
 };
 
-template <int N, int T, int C>
-struct PairLauncher {
-  template <int SP>
-  static hipError_t go(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
-                       double2* out, double2* ws, unsigned* flags) {
-    const void* k = reinterpret_cast<const void*>(&k_adaptive_pair<N, T, SP, C>);
-    const long long grid = 16 * ((batch + 7) / 8);
-    int per_cu = 0, dev = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, T, 0);
-    if (e == hipSuccess) e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return e;
-    if ((long long)per_cu * cus < grid) return hipErrorCooperativeLaunchTooLarge;
-    Ops ops = o;
-    AdaptiveParams prm = p;
-    void* args[] = {&ops, &prm, &in, &out, &ws, &flags, &batch};
-    return hipLaunchCooperativeKernel(k, dim3((unsigned)grid), dim3(T), args, 0, s);
-  }
-  static hipError_t adaptive(hipStream_t s, const Ops& o, const AdaptiveParams& p, long long batch, const double2* in,
-                             double2* out, double2* ws, unsigned* flags) {
-    if (o.gain == GAIN_NONE) return go<1>(s, o, p, batch, in, out, ws, flags);
-    if (o.gain == GAIN_SATURABLE && o.simp_flat) return go<2>(s, o, p, batch, in, out, ws, flags);
-    return hipErrorNotSupported;
-  }
-  static constexpr Schedule sched = {T, C, 1, &adaptive};
-};
 
 }  // namespace gn
 
@@ -722,10 +742,7 @@ struct PairLauncher {
   namespace gn {                            \
   const Schedule* schedule_##NN##_##ID() { return &SchedLauncher<NN, TT, CC>::sched; } \
   }
-#define GN_DEFINE_PAIR_SCHEDULE(NN, TT, CC, ID) \
-  namespace gn {                                 \
-  const Schedule* schedule_##NN##_##ID() { return &PairLauncher<NN, TT, CC>::sched; } \
-  }
+
 
 #define GN_DEFINE_GRID(NN, TT, CC) \
   namespace gn {                   \

@@ -55,8 +55,7 @@ def test_schedules_describe_valid_plans():
         assert len(sc) >= 2 and [s["index"] for s in sc] == list(range(len(sc)))
         descs = set()
         for s in sc:
-            desc = s["desc"][5:] if s["desc"].startswith("pair ") else s["desc"]
-            t_part, n_part, r_part, tile = desc.split(" ")
+            t_part, n_part, r_part, tile = s["desc"].split(" ")
             T = int(t_part[1:])
             assert T == s["threads"] and T % 64 == 0 and tile.startswith("tile")
             for length, plan in ((n, n_part[2:]), (n // 2, r_part[2:])):
@@ -74,9 +73,7 @@ def test_schedules_describe_valid_plans():
     assert _lib.schedules(16384)[0]["desc"].startswith("T1024 N:32L.32L.16 ")
     assert _lib.schedules(16384)[1]["desc"].startswith("T1024 N:16.16.4.16 ")
     assert _lib.schedules(4096)[0]["desc"] == "T512 N:8.8.8.8 R:4.4.4.4.2.4 tile4"
-    # each tunable grid also has a two-workgroup (pair) schedule
-    for n in (4096, 8192, 16384):
-        assert sum(s["desc"].startswith("pair ") for s in _lib.schedules(n)) == 1
+
     for n in (256, 1024):
         assert len(_lib.schedules(n)) == 1
     assert _lib.schedules(32768)[0]["desc"] == "T1024 two-level"

@@ -137,37 +137,60 @@ def test_schedule_selection_errors(lib):
     assert ctx.get_schedule() == 0
 
 
-def _pair_index(lib, N):
-    return next(s["index"] for s in lib.schedules(N) if s["desc"].startswith("pair "))
-
-
-def test_pair_schedule_equals_default_bit_for_bit(lib):
-    """Two workgroups per instance (half and fine steps beside the coarse
-    step) compute the same operations in the same order as one workgroup:
-    outputs, attempt logs and results are equal bit for bit, with traces."""
+def test_pairing_equals_one_workgroup_bit_for_bit(lib):
+    """Two workgroups per instance (the coarse step beside the half and fine
+    steps; on by default when the batch leaves room) compute the same
+    operations in the same order as one workgroup: outputs, attempt logs,
+    traces and results are equal bit for bit, for the saturable-gain
+    amplifier at 16384 samples (3 chirps) and the Kerr soliton at 4096
+    (rk4ip and ss_symmetric, one field: a plain Fibre run)."""
     N = 16384
     ctx, fib, A, refs, logs = amplifier_batch_ctx(lib, N)
     params = adaptive_params(lib, fib, 1e-6, traces=4)
     cap = max(len(l) for l in logs) + 8
-    ctx.set_schedule(0)
+    ctx.set_pairing(False)
     one = ctx.propagate_adaptive(0, A, params, max_traces=8, log_cap=cap)
-    ctx.set_schedule(_pair_index(lib, N))
+    t_one = ctx.last_kernel_ms()
+    ctx.set_pairing(True)
     two = ctx.propagate_adaptive(0, A, params, max_traces=8, log_cap=cap)
-    print(f"pair vs default at {N}: kernel {ctx.last_kernel_ms():.1f} ms", flush=True)
-    assert np.array_equal(one["A"], two["A"])
-    for key in ("log_h", "log_delta", "log_accept", "traces", "trace_z"):
-        if key in one:
-            assert np.array_equal(np.asarray(one[key]), np.asarray(two[key])), key
-    for r1, r2 in zip(one["results"], two["results"]):
-        assert r1 == r2
+    print(f"amplifier {N}: one workgroup {t_one:.1f} ms, two {ctx.last_kernel_ms():.1f} ms", flush=True)
+    _same(one, two)
     for i, ref in enumerate(refs):
         assert two["results"][i]["n_attempts"] == len(logs[i])
         assert rel_l2(two["A"][i], ref) <= FIELD_TOL
+    from helpers import soliton_c1
+    dom, A1, fib1 = soliton_c1(4096)
+    lin, nl = oracle_ops(dom, beta=fib1["beta"], gamma=fib1["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    fib1 = dict(fib1, length=fib1["length"] / 8)
+    for method, eta in ((0, 5), (1, 3)):
+        p = adaptive_params(lib, fib1, 1e-6, traces=3, eta=eta)
+        ctx.set_pairing(False)
+        one = ctx.propagate_adaptive(method, A1, p, max_traces=4, log_cap=4000)
+        t_one = ctx.last_kernel_ms()
+        ctx.set_pairing(True)
+        two = ctx.propagate_adaptive(method, A1, p, max_traces=4, log_cap=4000)
+        print(f"Kerr 4096 method {method}: {two['results'][0]['n_attempts']} attempts, one workgroup {t_one:.1f} ms, "
+              f"two {ctx.last_kernel_ms():.1f} ms", flush=True)
+        assert two["results"][0]["status"] == 0
+        _same(one, two)
 
 
-def test_pair_schedule_falls_back_when_the_batch_does_not_fit(lib):
+def _same(one, two):
+    """Outputs, results, and the written parts of the attempt logs and traces."""
+    assert np.array_equal(one["A"], two["A"])
+    for i, (r1, r2) in enumerate(zip(one["results"], two["results"])):
+        assert r1 == r2
+        na, nt = int(r1["n_attempts"]), int(r1["n_traces"])
+        for key in ("log_h", "log_delta", "log_accept"):
+            assert np.array_equal(one[key][i, :na], two[key][i, :na]), key
+        for key in ("traces", "trace_z", "step_z", "step_h"):
+            assert np.array_equal(one[key][i, :nt], two[key][i, :nt]), key
+
+
+def test_pairing_falls_back_when_the_batch_does_not_fit(lib):
     """A batch whose workgroups cannot all be resident (two per instance)
-    runs the grid's default kernel: the same results as schedule 0."""
+    runs one workgroup per instance: the same results as pairing off."""
     N = 8192
     dom, A, fib, ref, log = c2_reference(N)
     lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], self_steepening=True, use_all=True)
@@ -175,9 +198,9 @@ def test_pair_schedule_falls_back_when_the_batch_does_not_fit(lib):
     fib = dict(fib, length=2e-6)
     params = adaptive_params(lib, fib, 1e-6)
     batch = np.stack([A * (1 + 1e-4 * k) for k in range(300)])
-    ctx.set_schedule(0)
+    ctx.set_pairing(False)
     one = ctx.propagate_adaptive(0, batch, params)
-    ctx.set_schedule(_pair_index(lib, N))
+    ctx.set_pairing(True)
     two = ctx.propagate_adaptive(0, batch, params)
     assert np.array_equal(one["A"], two["A"])
     assert all(r["status"] == 0 for r in two["results"])
