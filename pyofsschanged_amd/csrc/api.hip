@@ -961,6 +961,7 @@ int gnlse_last_kernel_ms(gnlse_ctx* c, double* ms) {
 // ---------------------------------------------------------------------------
 struct gnlse_ensemble {
   int device = 0, method = 0, kind = NL_KERR, ngroups = 0;
+  bool gain = false;  // some group's operators carry a gain
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -993,7 +994,7 @@ static void drop_graph(gnlse_ensemble* e) {
 // The launch sequence of one ensemble run: the propagation kernel, then one
 // coherence reduction per group of two or more realisations.
 static hipError_t ensemble_enqueue(gnlse_ensemble* e, const EnsembleGroup* dgroups) {
-  hipError_t r = e->tab->run(e->stream, e->method, e->kind, dgroups, e->ngroups, e->blocks);
+  hipError_t r = e->tab->run(e->stream, e->method, e->kind, e->gain, dgroups, e->ngroups, e->blocks);
   if (r != hipSuccess) return r;
   for (int i = 0; i < e->ngroups; ++i) {
     const EnsembleGroup& g = e->groups[i];
@@ -1089,6 +1090,7 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
     ress += (size_t)batch[i];
     all_kerr = all_kerr && ctxs[i]->ops.nl_kind == NL_KERR;
     all_sp = all_sp && ctxs[i]->ops.nl_kind == NL_ALL && ctxs[i]->ops.gain == GAIN_NONE && !ctxs[i]->ops.wdm;
+    e->gain = e->gain || ctxs[i]->ops.gain != GAIN_NONE;
   }
   // the spectral-domain RK4IP when every group has the full operator without gain
   e->kind = all_kerr ? NL_KERR : all_sp ? NL_ALL : NL_GENERIC;

@@ -57,7 +57,9 @@ struct EnsembleShape {
   static constexpr int LDS_DOUBLES = a > b ? (a > c ? a : c) : (b > c ? b : c);
 };
 
-template <int T, int METHOD, int K>
+// X: further Dev CFG bits of every grid (kCfgKerrFree when the groups are
+// Kerr without gain).
+template <int T, int METHOD, int K, int X = 0>
 __global__ void __launch_bounds__(T) k_ensemble(const EnsembleGroup* __restrict__ groups, int ngroups) {
   __shared__ double lds[EnsembleShape<T>::LDS_DOUBLES];
   __shared__ double red[T / 64 + 2];
@@ -69,13 +71,13 @@ __global__ void __launch_bounds__(T) k_ensemble(const EnsembleGroup* __restrict_
   const EnsembleGroup& g = groups[gi];
   const size_t b = (size_t)(w - g.first);
   if (g.n == 4096)
-    adaptive_body<4096, T, METHOD, K, false, 0, ens_cfg<4096, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
+    adaptive_body<4096, T, METHOD, K, false, 0, ens_cfg<4096, T>() | X>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
                                                                      g.spec);
   else if (g.n == 8192)
-    adaptive_body<8192, T, METHOD, K, false, 0, ens_cfg<8192, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
+    adaptive_body<8192, T, METHOD, K, false, 0, ens_cfg<8192, T>() | X>(lds, red, g.ops, g.prm, b, g.in, g.out, g.ws,
                                                                      g.spec);
   else
-    adaptive_body<16384, T, METHOD, K, false, 0, ens_cfg<16384, T>()>(lds, red, g.ops, g.prm, b, g.in, g.out,
+    adaptive_body<16384, T, METHOD, K, false, 0, ens_cfg<16384, T>() | X>(lds, red, g.ops, g.prm, b, g.in, g.out,
                                                                        g.ws, g.spec);
 }
 
@@ -129,19 +131,26 @@ __global__ void __launch_bounds__(BS) k_coherence(const double2* __restrict__ sp
 
 struct EnsembleTable {
   int t;
-  hipError_t (*run)(hipStream_t, int method, int kind, const EnsembleGroup* groups, int ngroups, long long blocks);
+  hipError_t (*run)(hipStream_t, int method, int kind, bool gain, const EnsembleGroup* groups, int ngroups,
+                    long long blocks);
 };
 
 template <int T>
 struct EnsembleLauncher {
-  static hipError_t run(hipStream_t s, int method, int kind, const EnsembleGroup* groups, int ngroups,
+  // gain: some group's operators carry a gain
+  static hipError_t run(hipStream_t s, int method, int kind, bool gain, const EnsembleGroup* groups, int ngroups,
                         long long blocks) {
     if (method == M_RK4IP) {
       if (kind == NL_ALL) hipLaunchKernelGGL((k_ensemble_sp<T>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
+      else if (kind == NL_KERR && !gain)
+        hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_KERR, kCfgKerrFree>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
       else if (kind == NL_KERR) hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_KERR>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
       else hipLaunchKernelGGL((k_ensemble<T, M_RK4IP, NL_GENERIC>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
     } else {
-      if (kind == NL_KERR)
+      if (kind == NL_KERR && !gain)
+        hipLaunchKernelGGL((k_ensemble<T, M_SS_SYMMETRIC, NL_KERR, kCfgKerrFree>), dim3(blocks), dim3(T), 0, s, groups,
+                           ngroups);
+      else if (kind == NL_KERR)
         hipLaunchKernelGGL((k_ensemble<T, M_SS_SYMMETRIC, NL_KERR>), dim3(blocks), dim3(T), 0, s, groups, ngroups);
       else
         hipLaunchKernelGGL((k_ensemble<T, M_SS_SYMMETRIC, NL_GENERIC>), dim3(blocks), dim3(T), 0, s, groups, ngroups);

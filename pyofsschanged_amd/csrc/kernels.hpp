@@ -568,7 +568,9 @@ hipError_t launch_pair(hipStream_t s, const Ops& o, const AdaptiveParams& p, lon
     if ((ops).gain == GAIN_YB) {                                                                        \
       if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, true, 0, CC>), __VA_ARGS__);     \
       else hipLaunchKernelGGL((KERNEL, NL_GENERIC, true, 0, CC>), __VA_ARGS__);                         \
-    } else if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, CC>), __VA_ARGS__); \
+    } else if ((ops).nl_kind == NL_KERR && (ops).gain == GAIN_NONE)                                     \
+      hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, (CC) | kCfgKerrFree>), __VA_ARGS__);                   \
+    else if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, CC>), __VA_ARGS__); \
     else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, false, 0, CC>), __VA_ARGS__); \
     else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false, 0, CC>), __VA_ARGS__);                          \
   } while (0)
@@ -710,9 +712,10 @@ struct Launcher {
       if (method == M_RK4IP && o.nl_kind == NL_ALL && o.gain == GAIN_SATURABLE && o.simp_flat)
         return launch_pair<N, T, M_RK4IP, NL_ALL, 2, C>(s, o, p, batch, in, out, ws, flags);
       if (o.nl_kind == NL_KERR && o.gain == GAIN_NONE) {
-        if (method == M_RK4IP) return launch_pair<N, T, M_RK4IP, NL_KERR, 0, C>(s, o, p, batch, in, out, ws, flags);
+        if (method == M_RK4IP)
+          return launch_pair<N, T, M_RK4IP, NL_KERR, 0, C | kCfgKerrFree>(s, o, p, batch, in, out, ws, flags);
         if (method == M_SS_SYMMETRIC)
-          return launch_pair<N, T, M_SS_SYMMETRIC, NL_KERR, 0, C>(s, o, p, batch, in, out, ws, flags);
+          return launch_pair<N, T, M_SS_SYMMETRIC, NL_KERR, 0, C | kCfgKerrFree>(s, o, p, batch, in, out, ws, flags);
       }
       return hipErrorNotSupported;
     }

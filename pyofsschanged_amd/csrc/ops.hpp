@@ -260,7 +260,14 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
 // that of the N/2-point transform of the Raman convolution, bits 8-9 the
 // block of the stage passes of rk4ip_sp (0: 4 points, 1: 2, 2: 8). A
 // lane-pair plan in either transform renumbers the threads of the whole
-// kernel (every transform and pass follows one ownership rule).
+// kernel (every transform and pass follows one ownership rule). Bit 10
+// (kCfgNoGain): the launch has checked that the operators carry no gain, so
+// the linear calls compile without the saturable-gain branch; bit 12
+// (kCfgLine): rk4ip runs written out stage by stage (rk4ip_line). The Kerr
+// launches without gain set both (kCfgKerrFree): the runtime gain branch and
+// the merging paths of the 7-stage loop cost the 16384-point Kerr kernels
+// half their speed in register spills.
+constexpr int kCfgNoGain = 0x400, kCfgLine = 0x1000, kCfgKerrFree = kCfgNoGain | kCfgLine;
 template <int N, int T, int KIND, bool YB = false, int CFG = 0>
 struct Dev {
   static constexpr int RENUM = ((CFG & 8) || (CFG & 0x80)) ? 16 : 0;
@@ -270,6 +277,8 @@ struct Dev {
   static constexpr int P = N / T;
   static constexpr double INVN = 1.0 / N;  // exact: N is a power of two
   static constexpr int NW = T / 64;
+  static constexpr bool NOGAIN = (CFG & kCfgNoGain) != 0;
+  static constexpr bool LINE = (CFG & kCfgLine) != 0;
   double* lds;
   double* red;
   const Ops& op;
@@ -388,6 +397,8 @@ struct Dev {
   __device__ __forceinline__ void linear(double2 (&x)[P], const double2* __restrict__ E, double h) {
     if constexpr (YB) {
       spectral(x, MUL_YB, E, h);
+    } else if (NOGAIN) {
+      spectral(x, MUL_ARRAY, E, 0.0);
     } else if (op.gain == GAIN_SATURABLE) {
       double e = 0.0;
 #pragma unroll
@@ -632,6 +643,69 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N /
         x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
       }
     }
+  }
+}
+
+// The steps of rk4ip above written out in order, one call site per
+// operator call (four linear calls, four N(x)), for kernels whose N(x) is
+// pointwise (Kerr). The same operations on the same slots; the only
+// difference from the loop is how the compiler schedules and allocates it:
+// 16384 points at 1024 threads spill 36 B per thread in this form against
+// 436 B in the loop, whose merging paths keep extra copies of the field.
+template <int N, int T, int K, bool YB, int CFG>
+__device__ __forceinline__ void rk4ip_line(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in,
+                                           const double2* E, double h, double2* ws, double2* nA, bool nA_ready,
+                                           bool nA_store) {
+  constexpr int P = N / T;
+  double2* sAI = ws + S_AI * N;
+  double2* sK0 = ws + S_K0 * N;
+  double2* sK1 = ws + S_K1 * N;
+  double2* sT = ws + S_T * N;
+  double2* sU = ws + S_U * N;
+  const double hh = 0.5 * h;
+  d.load(x, in);  // A_I = L(A)
+  d.linear(x, E, hh);
+  d.store(sAI, x);
+  if (nA_ready) {  // k0 = L(h N(A))
+    d.load(x, nA);
+  } else {
+    d.load(x, in);
+    d.nonlin(x, sU, sT);
+    if (nA_store) d.store(nA, x);
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
+  d.linear(x, E, hh);
+  d.store(sK0, x);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = cadd(d.mine(sAI)[T * k], cscale(x[k], 0.5));  // k1 = h N(A_I + k0/2)
+  d.nonlin(x, sU, sT);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
+  d.store(sK1, x);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = cadd(d.mine(sAI)[T * k], cscale(x[k], 0.5));  // k2 = h N(A_I + k1/2)
+  d.nonlin(x, sU, sT);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int i = d.idx(k);
+    const double2 k2 = cscale(x[k], h);
+    const double2 ai = sAI[i];
+    const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], k2), 2.0));
+    sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+    x[k] = cadd(ai, k2);                                    // A_I + k2
+  }
+  d.linear(x, E, hh);  // k3 = h N(L(A_I + k2))
+  d.nonlin(x, sU, sT);
+#pragma unroll
+  for (int k = 0; k < P; ++k) x[k] = cscale(x[k], h);
+  d.store(sK1, x);
+  d.load(x, sK0);  // A' = L(A_I + (k0 + 2(k1+k2))/6) + k3/6
+  d.linear(x, E, hh);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double2 k3 = sK1[d.idx(k)];
+    x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
   }
 }
 
@@ -1091,7 +1165,9 @@ template <int METHOD, int N, int T, int K, bool YB, int CFG>
 __device__ __forceinline__ void integrator(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N / T], const double2* in, const double2* E,
                                            double h, double2* ws, double2* nA = nullptr, bool nA_ready = false,
                                            bool nA_store = false) {
-  if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);
+  if constexpr (METHOD == M_RK4IP && Dev<N, T, K, YB, CFG>::LINE)
+    rk4ip_line(d, x, in, E, h, ws, nA, nA_ready, nA_store);
+  else if constexpr (METHOD == M_RK4IP) rk4ip(d, x, in, E, h, ws, nA, nA_ready, nA_store);
   else if constexpr (METHOD == M_SS_SYM_RK4) ss_sym_rk4(d, x, in, E, h, ws);
   else ss_symmetric(d, x, in, E, h, ws);
 }

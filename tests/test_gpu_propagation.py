@@ -227,6 +227,45 @@ def test_c4_physics_fixed_steps(lib, N):
         assert rel_l2(tr[0, k], ref_As[k]) <= FIELD_TOL
 
 
+@pytest.mark.parametrize("N,length", [(4096, 1e-3), (16384, 2e-4)])
+def test_kerr_amplifier_adaptive_and_fixed(lib, N, length):
+    """The saturable gain with the Kerr operator alone (no Raman): the
+    time-domain Kerr kernels that keep the gain branch (the gain-free Kerr
+    launches compile without it). Adaptive rk4ip of two chirps in one launch
+    (the oracle's attempt sequences), then fixed rk4ip steps."""
+    refs, logs, inputs = [], [], []
+    for C in (-1.0, 1.5):
+        dom, A, fib = amplifier_c4(N, C)
+        fib = dict(fib, length=length, use_all=False)
+        ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+        refs.append(ref)
+        logs.append(f.stepper.attempt_log)
+        inputs.append(A)
+    amp = oracle_amplifier(dom, length, **fib["amplifier"])
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], amplifier=amp)
+    ctx = engine_context(dom, lin, nl, 0, amplifier=amp)
+    out = ctx.propagate_adaptive(0, np.stack(inputs), adaptive_params(lib, fib, 1e-6),
+                                 log_cap=max(len(l) for l in logs) + 8)
+    for i, (ref, log) in enumerate(zip(refs, logs)):
+        r = out["results"][i]
+        err = rel_l2(out["A"][i], ref)
+        print(f"Kerr amplifier N={N} chirp {i}: attempts {r['n_attempts']} (oracle {len(log)}), rel L2 {err:.3e}",
+              flush=True)
+        assert r["status"] == 0
+        assert r["n_attempts"] == len(log)
+        assert np.array_equal(out["log_accept"][i, :len(log)], np.array([a[2] for a in log], dtype=np.uint8))
+        assert err <= FIELD_TOL
+    steps = 40
+    f = O.Fibre(length=length, beta=fib["beta"], gamma=fib["gamma"], method="rk4ip", total_steps=steps,
+                small_signal_gain=fib["amplifier"]["gain"], E_sat=fib["amplifier"]["E_sat"],
+                lamb0=fib["amplifier"]["lamb0"], bandwidth=fib["amplifier"]["bandwidth"])
+    ref = f(dom, inputs[0].copy())
+    got, _, _ = ctx.propagate_fixed(0, inputs[0], length / steps, steps)
+    err = rel_l2(got, ref)
+    print(f"Kerr amplifier N={N} fixed: rel L2 {err:.3e}", flush=True)
+    assert err <= FIELD_TOL
+
+
 def test_adaptive_nan_input_raises_small_step(lib):
     """stepper.py:234-237, 349-351: NaN fields are rejected until h < 1e-37."""
     dom, A, fib = soliton_c1(1024)
