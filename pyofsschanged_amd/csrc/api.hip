@@ -68,8 +68,9 @@ static std::string schedule_desc(long long n, int t, int cfg) {
   };
   const int tb = (cfg >> 8) & 3;
   const int tile = tb == 1 ? 2 : tb == 2 ? 8 : 4;
+  const bool sp = cfg & kCfgSpUnroll, ra = cfg & kCfgRamanUnroll;
   return "T" + std::to_string(t) + " N:" + plan(n, (cfg & 15) | renum) + " R:" + plan(n / 2, ((cfg >> 4) & 15) | renum) +
-         " tile" + std::to_string(tile);
+         " tile" + std::to_string(tile) + (sp && ra ? " unrolled" : sp ? " passes unrolled" : ra ? " raman unrolled" : "");
 }
 }  // namespace gn
 

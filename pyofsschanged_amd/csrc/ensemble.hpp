@@ -41,12 +41,13 @@ struct EnsembleGroup {
   long long n, first, count;
 };
 
-// Schedule (Dev's CFG) of a grid inside an ensemble launch: the lane-pair
-// N-point plan of the 8192- and 16384-sample grids' own launches where the
-// points per thread allow it.
+// Schedule (Dev's CFG) of a grid inside an ensemble launch: the default
+// schedule of the 8192- and 16384-sample grids' own launches (lane-pair
+// N-point plan, unrolled spectral-domain passes; 8-point stage blocks at
+// 8192) where the points per thread allow it.
 template <int N, int T>
 constexpr int ens_cfg() {
-  return (N >= 8192 && N / T <= 16) ? 8 : 0;
+  return (N >= 8192 && N / T <= 16) ? (8 | kCfgUnrolled | (N == 8192 && T == 512 ? 0x200 : 0)) : 0;
 }
 
 template <int T>

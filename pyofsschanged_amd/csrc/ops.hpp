@@ -266,8 +266,15 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
 // (kCfgLine): rk4ip runs written out stage by stage (rk4ip_line). The Kerr
 // launches without gain set both (kCfgKerrFree): the runtime gain branch and
 // the merging paths of the 7-stage loop cost the 16384-point Kerr kernels
-// half their speed in register spills.
+// half their speed in register spills. Bit 13 (kCfgSpUnroll): the twelve
+// passes of rk4ip_sp unrolled (each pass compiled for its stage and phase);
+// bit 14 (kCfgRamanUnroll): the two transforms of raman_real at two call
+// sites. Both trade code size for fewer register copies where the loops'
+// paths merge: 8192 points at 512 threads spill 100 B per thread instead of
+// 264 B, and the headline attempt drops from 501 to 443 us (with 8-point
+// stage blocks).
 constexpr int kCfgNoGain = 0x400, kCfgLine = 0x1000, kCfgKerrFree = kCfgNoGain | kCfgLine;
+constexpr int kCfgSpUnroll = 0x2000, kCfgRamanUnroll = 0x4000, kCfgUnrolled = kCfgSpUnroll | kCfgRamanUnroll;
 template <int N, int T, int KIND, bool YB = false, int CFG = 0>
 struct Dev {
   static constexpr int RENUM = ((CFG & 8) || (CFG & 0x80)) ? 16 : 0;
@@ -279,6 +286,8 @@ struct Dev {
   static constexpr int NW = T / 64;
   static constexpr bool NOGAIN = (CFG & kCfgNoGain) != 0;
   static constexpr bool LINE = (CFG & kCfgLine) != 0;
+  static constexpr int SP_UNROLL = (CFG & kCfgSpUnroll) ? 12 : 1;
+  static constexpr int RAMAN_UNROLL = (CFG & kCfgRamanUnroll) ? 2 : 1;
   double* lds;
   double* red;
   const Ops& op;
@@ -942,7 +951,7 @@ __device__ __forceinline__ void raman_real(Dev<N, T, K, YB, CFG>& d, double2 (&x
   double2 z[P2];
 #pragma unroll
   for (int j = 0; j < P2; ++j) z[j] = l2[t + T * j];
-#pragma unroll 1
+#pragma unroll Dev<N, T, K, YB, CFG>::RAMAN_UNROLL
   for (int it = 0; it < 2; ++it) {
     F2::plus(z, d.lds, tw2);
     if (it == 0) {  // pair Z_k with Z_M-k, multiply, fold back to M points
@@ -1054,8 +1063,9 @@ __device__ __forceinline__ void rk4ip_sp(Dev<N, T, K, YB, CFG>& d, double2 (&x)[
     if (inS) pw_sp_entry<P, T>(x, d.mine(E), esq, d.mine(inS), d.mine(sAIs));
   }
   bool skip = nA_ready;  // the next pass has no transform (Nt(A) was loaded)
-#pragma unroll 1
-  for (int p = p0; p < 12; ++p) {
+#pragma unroll Dev<N, T, K, YB, CFG>::SP_UNROLL
+  for (int p = 0; p < 12; ++p) {
+    if (p < p0) continue;
     const int st = p / 3 + 1;
     const int ph = p % 3;
     if (ph == 0) {  // x R(x) = (1-f_R) x |x|^2 + f_R x conv, x parked in sU

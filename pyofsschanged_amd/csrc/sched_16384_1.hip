@@ -1,5 +1,6 @@
 // Schedule 1 of the 16384-sample grid's spectral-domain adaptive RK4IP
-// kernels, one of the autotuner's candidates: 1024 threads, the plain plans
-// (no lane-pair pass).
+// kernels, one of the autotuner's candidates: 1024 threads, lane-pair passes
+// in the 16384-point transform, the passes of rk4ip_sp and raman_real in
+// their loops.
 #include "kernels.hpp"
-GN_DEFINE_SCHEDULE(16384, 1024, 0x00, 1)
+GN_DEFINE_SCHEDULE(16384, 1024, 0x08, 1)

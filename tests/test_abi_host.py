@@ -55,7 +55,9 @@ def test_schedules_describe_valid_plans():
         assert len(sc) >= 2 and [s["index"] for s in sc] == list(range(len(sc)))
         descs = set()
         for s in sc:
-            t_part, n_part, r_part, tile = s["desc"].split(" ")
+            parts = s["desc"].split(" ")
+            t_part, n_part, r_part, tile = parts[:4]
+            assert parts[4:] in ([], ["unrolled"], ["passes", "unrolled"], ["raman", "unrolled"]), s["desc"]
             T = int(t_part[1:])
             assert T == s["threads"] and T % 64 == 0 and tile.startswith("tile")
             for length, plan in ((n, n_part[2:]), (n // 2, r_part[2:])):
@@ -66,12 +68,12 @@ def test_schedules_describe_valid_plans():
                     assert (r == 2 * P) if lane else (P % r == 0), s["desc"]
             descs.add(s["desc"])
         assert len(descs) == len(sc)  # no duplicate schedule
-    # defaults: a lane-pair pass in the 8192- and 16384-point transforms;
-    # schedule 1 keeps the plain plans
-    assert _lib.schedules(8192)[0]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile4"
-    assert _lib.schedules(8192)[1]["desc"] == "T512 N:16.16.2.16 R:8.8.8.8 tile4"
-    assert _lib.schedules(16384)[0]["desc"].startswith("T1024 N:32L.32L.16 ")
-    assert _lib.schedules(16384)[1]["desc"].startswith("T1024 N:16.16.4.16 ")
+    # defaults: a lane-pair pass in the 8192- and 16384-point transforms and
+    # unrolled passes; schedule 1 keeps the passes in their loops
+    assert _lib.schedules(8192)[0]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile8 unrolled"
+    assert _lib.schedules(8192)[1]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile4"
+    assert _lib.schedules(16384)[0]["desc"] == "T1024 N:32L.32L.16 R:8.8.8.2.8 tile4 unrolled"
+    assert _lib.schedules(16384)[1]["desc"] == "T1024 N:32L.32L.16 R:8.8.8.2.8 tile4"
     assert _lib.schedules(4096)[0]["desc"] == "T512 N:8.8.8.8 R:4.4.4.4.2.4 tile4"
 
     for n in (256, 1024):
