@@ -43,11 +43,13 @@ struct EnsembleGroup {
 
 // Schedule (Dev's CFG) of a grid inside an ensemble launch: the default
 // schedule of the 8192- and 16384-sample grids' own launches (lane-pair
-// N-point plan, unrolled spectral-domain passes; 8-point stage blocks at
-// 8192) where the points per thread allow it.
+// N-point plan, unrolled spectral-domain passes; stage blocks of 8 points
+// at 8192 and 2 at 16384) where the points per thread allow it.
 template <int N, int T>
 constexpr int ens_cfg() {
-  return (N >= 8192 && N / T <= 16) ? (8 | kCfgUnrolled | (N == 8192 && T == 512 ? 0x200 : 0)) : 0;
+  return (N >= 8192 && N / T <= 16)
+             ? (8 | kCfgUnrolled | (N == 8192 && T == 512 ? 0x200 : 0) | (N == 16384 && T == 1024 ? 0x100 : 0))
+             : 0;
 }
 
 template <int T>

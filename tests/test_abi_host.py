@@ -72,7 +72,7 @@ def test_schedules_describe_valid_plans():
     # unrolled passes; schedule 1 keeps the passes in their loops
     assert _lib.schedules(8192)[0]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile8 unrolled"
     assert _lib.schedules(8192)[1]["desc"] == "T512 N:32L.16.16 R:8.8.8.8 tile4"
-    assert _lib.schedules(16384)[0]["desc"] == "T1024 N:32L.32L.16 R:8.8.8.2.8 tile4 unrolled"
+    assert _lib.schedules(16384)[0]["desc"] == "T1024 N:32L.32L.16 R:8.8.8.2.8 tile2 unrolled"
     assert _lib.schedules(16384)[1]["desc"] == "T1024 N:32L.32L.16 R:8.8.8.2.8 tile4"
     assert _lib.schedules(4096)[0]["desc"] == "T512 N:8.8.8.8 R:4.4.4.4.2.4 tile4"
 
