@@ -196,7 +196,7 @@ def ensemble_measurement():
     replayed from a hipGraph, |g12| per grid on the device."""
     from pyofsschanged_amd import workloads
     ens, setups = workloads.noise_ensemble()
-    threads = ens.tune(0)
+    threads = ens.tune(0, trial_attempts=32)
     ens.run(graph=True)  # capture + first replay
     ms = ens.run(graph=True)
     att = {s["N"]: float(np.mean([r["n_attempts"] for r in ens.output(g)["results"]])) for g, s in enumerate(setups)}
@@ -208,6 +208,7 @@ def ensemble_measurement():
             "realisations": {str(s["N"]): len(s["seeds"]) for s in setups},
             "mean_attempts": {str(k): v for k, v in att.items()},
             "launch_ms": ms, "propagations_per_s": total / (1e-3 * ms), "threads_per_realisation": threads,
+            "threads_trial": "512 and 1024 threads per realisation, 32 replayed steps of h0 for every realisation",
             "graph": True}
 
 

@@ -574,7 +574,7 @@ class Ensemble:
         with self.lock:
             _check(self.lib, getattr(self.lib, name)(self._h, *args), name)
 
-    def tune(self, threads=0, trial_attempts=3):
+    def tune(self, threads=0, trial_attempts=32):
         """threads 512 or 1024 fixes the launch shape; 0 times both on a short
         trial and keeps the faster. Returns the chosen thread count."""
         c = ctypes.c_int()
