@@ -311,7 +311,9 @@ int gnlse_tune_adaptive(gnlse_ctx* ctx, int64_t batch, const gnlse_adaptive_para
  * realisation; this replaces that loop for the stepping path). Groups are up
  * to four sets of realisations, each on the grid (4096, 8192 or 16384) and
  * operators of one context; every realisation runs Stepper.adaptive_stepper
- * (stepper.py:247-376) with its own step sizes, all in one launch. After the
+ * (stepper.py:247-376) with its own step sizes, all in one launch per thread
+ * count (groups at a second thread count run in a second launch beside the
+ * first, in one graph when replayed). After the
  * propagation the spectral coherence of each group of M >= 2 realisations,
  * g12 = | |sum a_i|^2 - sum |a_i|^2 | / ((M-1) sum |a_i|^2) with a_i the
  * pyofss spectrum fft(A_i) (field.py:109-121), is formed on the device.
@@ -319,10 +321,16 @@ int gnlse_tune_adaptive(gnlse_ctx* ctx, int64_t batch, const gnlse_adaptive_para
 typedef struct gnlse_ensemble gnlse_ensemble;
 int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroups, gnlse_ctx* const* ctxs,
                           const int64_t* batch, gnlse_ensemble** out);
-/* threads = 512 or 1024 selects the launch shape; 0 times each candidate on
- * trial_attempts replayed steps of h0 per realisation and keeps the fastest. */
+/* threads = 512 or 1024 selects the launch shape of every group; 0 times
+ * each group alone at each candidate on trial_attempts (1..64) replayed steps
+ * of h0 per realisation and keeps the fastest for that group. chosen (and
+ * gnlse_ensemble_threads) report the thread count of the largest grid's
+ * group; gnlse_ensemble_group_threads that of any group (caller order). */
 int gnlse_ensemble_tune(gnlse_ensemble* e, int threads, int trial_attempts, int* chosen);
 int gnlse_ensemble_threads(gnlse_ensemble* e, int* threads);
+int gnlse_ensemble_group_threads(gnlse_ensemble* e, int group, int* threads);
+/* sets one group's thread count (512 or 1024) by hand */
+int gnlse_ensemble_set_group_threads(gnlse_ensemble* e, int group, int threads);
 /* device pointers of a group's inputs (batch x N complex128), outputs and g12 */
 int gnlse_ensemble_device_ptrs(gnlse_ensemble* e, int group, double** d_in, double** d_out, double** d_g12);
 int gnlse_ensemble_set_input(gnlse_ensemble* e, int group, const double* A_in);

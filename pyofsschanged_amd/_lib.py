@@ -126,6 +126,8 @@ SIGNATURES = {
     "gnlse_ensemble_create": [_INT, ctypes.POINTER(AdaptiveParams), _INT, _P, _P, ctypes.POINTER(_P)],
     "gnlse_ensemble_tune": [_P, _INT, _INT, ctypes.POINTER(ctypes.c_int)],
     "gnlse_ensemble_threads": [_P, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_ensemble_group_threads": [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "gnlse_ensemble_set_group_threads": [_P, ctypes.c_int, ctypes.c_int],
     "gnlse_ensemble_device_ptrs": [_P, _INT, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)],
     "gnlse_ensemble_set_input": [_P, _INT, _P],
     "gnlse_ensemble_run": [_P, _INT],
@@ -575,8 +577,10 @@ class Ensemble:
             _check(self.lib, getattr(self.lib, name)(self._h, *args), name)
 
     def tune(self, threads=0, trial_attempts=32):
-        """threads 512 or 1024 fixes the launch shape; 0 times both on a short
-        trial and keeps the faster. Returns the chosen thread count."""
+        """threads 512 or 1024 fixes the launch shape of every group; 0 times
+        each group alone at both on a short trial and keeps the faster for
+        that group. Returns the thread count of the largest grid's group
+        (group_threads gives each group's)."""
         c = ctypes.c_int()
         self._call("gnlse_ensemble_tune", int(threads), int(trial_attempts), ctypes.byref(c))
         return c.value
@@ -586,6 +590,14 @@ class Ensemble:
         c = ctypes.c_int()
         self._call("gnlse_ensemble_threads", ctypes.byref(c))
         return c.value
+
+    def group_threads(self, group):
+        c = ctypes.c_int()
+        self._call("gnlse_ensemble_group_threads", int(group), ctypes.byref(c))
+        return c.value
+
+    def set_group_threads(self, group, threads):
+        self._call("gnlse_ensemble_set_group_threads", int(group), int(threads))
 
     def device_ptrs(self, group):
         a, b, g = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()

@@ -958,13 +958,25 @@ int gnlse_last_kernel_ms(gnlse_ctx* c, double* ms) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Ensembles: realisations on up to four grids in one launch (ensemble.hpp).
+// Ensembles: realisations on up to four grids (ensemble.hpp). The groups that
+// share a thread count run in one launch; a second thread count runs in a
+// second launch beside it, on a second stream (joined before the coherence
+// kernels), so each grid can keep the thread count the tuner finds fastest
+// for it.
 // ---------------------------------------------------------------------------
+// One launch of a run: its groups (launch order) with block offsets within it.
+struct EnsLaunch {
+  const EnsembleTable* tab = nullptr;
+  int ng = 0;
+  int64_t blocks = 0;
+  EnsembleGroup g[kEnsMaxGroups];
+};
+
 struct gnlse_ensemble {
   int device = 0, method = 0, kind = NL_KERR, ngroups = 0;
   bool gain = false;  // some group's operators carry a gain
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
   double last_ms = 0.0;
   gnlse_adaptive_params params{};
   // caller's group order; slot[i] = position of caller group i in launch order
@@ -974,7 +986,9 @@ struct gnlse_ensemble {
   int64_t blocks = 0;
   std::vector<EnsembleGroup> groups;  // launch order (largest grid first)
   DevBuf in, out, ws, spec, g12, res, dgroups, rp_h, rp_a;
-  const EnsembleTable* tab = nullptr;
+  const EnsembleTable* tabs[kEnsMaxGroups] = {};  // thread count of each group (launch order)
+  EnsLaunch launch[2];                            // the launches of a run; their tables in dgroups
+  int nlaunch = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -992,10 +1006,57 @@ static void drop_graph(gnlse_ensemble* e) {
   e->graph = nullptr;
 }
 
-// The launch sequence of one ensemble run: the propagation kernel, then one
-// coherence reduction per group of two or more realisations.
-static hipError_t ensemble_enqueue(gnlse_ensemble* e, const EnsembleGroup* dgroups) {
-  hipError_t r = e->tab->run(e->stream, e->method, e->kind, e->gain, dgroups, e->ngroups, e->blocks);
+// Split the groups by thread count into at most two launches (group order
+// kept within each) and upload their tables to d (kEnsMaxGroups entries per
+// launch). tabs gives each group's table; only groups with sel set take part.
+static int ens_plan(gnlse_ensemble* e, const std::vector<EnsembleGroup>& groups, const EnsembleTable* const* tabs,
+                    const bool* sel, DevBuf& d, EnsLaunch (&L)[2], int& nl) {
+  nl = 0;
+  for (int s = 0; s < e->ngroups; ++s) {
+    if (!sel[s]) continue;
+    int k = 0;
+    while (k < nl && L[k].tab != tabs[s]) ++k;
+    if (k == nl) {
+      L[k] = EnsLaunch();
+      L[k].tab = tabs[s];
+      ++nl;
+    }
+    EnsembleGroup g = groups[s];
+    g.first = L[k].blocks;
+    L[k].g[L[k].ng++] = g;
+    L[k].blocks += g.count;
+  }
+  RET_IF(ensure(d, 2 * kEnsMaxGroups * sizeof(EnsembleGroup)));
+  for (int k = 0; k < nl; ++k)
+    HIPCHK(hipMemcpyAsync((EnsembleGroup*)d.p + k * kEnsMaxGroups, L[k].g, L[k].ng * sizeof(EnsembleGroup),
+                          hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return GNLSE_OK;
+}
+
+// The launches of a run from the groups' current thread counts.
+static int ens_replan(gnlse_ensemble* e) {
+  bool all[kEnsMaxGroups];
+  for (int s = 0; s < kEnsMaxGroups; ++s) all[s] = true;
+  return ens_plan(e, e->groups, e->tabs, all, e->dgroups, e->launch, e->nlaunch);
+}
+
+// The launch sequence of one ensemble run: the propagation launches (a
+// second one forked onto stream2 and joined back), then one coherence
+// reduction per group of two or more realisations.
+static hipError_t ensemble_enqueue(gnlse_ensemble* e, const EnsLaunch* L, int nl, const EnsembleGroup* dbase) {
+  hipError_t r = hipSuccess;
+  if (nl > 1) {
+    r = hipEventRecord(e->ev_fork, e->stream);
+    if (r == hipSuccess) r = hipStreamWaitEvent(e->stream2, e->ev_fork, 0);
+  }
+  for (int k = 0; k < nl && r == hipSuccess; ++k)
+    r = L[k].tab->run(k == 0 ? e->stream : e->stream2, e->method, e->kind, e->gain, dbase + k * kEnsMaxGroups, L[k].ng,
+                      L[k].blocks);
+  if (nl > 1 && r == hipSuccess) {
+    r = hipEventRecord(e->ev_join, e->stream2);
+    if (r == hipSuccess) r = hipStreamWaitEvent(e->stream, e->ev_join, 0);
+  }
   if (r != hipSuccess) return r;
   for (int i = 0; i < e->ngroups; ++i) {
     const EnsembleGroup& g = e->groups[i];
@@ -1009,11 +1070,11 @@ static hipError_t ensemble_enqueue(gnlse_ensemble* e, const EnsembleGroup* dgrou
   return hipSuccess;
 }
 
-static int ensemble_time(gnlse_ensemble* e, const EnsembleGroup* dgroups, bool graph) {
+static int ensemble_time(gnlse_ensemble* e, const EnsLaunch* L, int nl, const EnsembleGroup* dbase, bool graph) {
   HIPCHK(hipEventRecord(e->ev0, e->stream));
   if (graph) HIPCHK(hipGraphLaunch(e->exec, e->stream));
   else {
-    hipError_t r = ensemble_enqueue(e, dgroups);
+    hipError_t r = ensemble_enqueue(e, L, nl, dbase);
     if (r != hipSuccess) return fail(GNLSE_ERR_HIP, std::string("ensemble launch: ") + hipGetErrorString(r));
   }
   HIPCHK(hipEventRecord(e->ev1, e->stream));
@@ -1033,9 +1094,12 @@ int gnlse_ensemble_destroy(gnlse_ensemble* e) {
   drop_graph(e);
   DevBuf* bufs[] = {&e->in, &e->out, &e->ws, &e->spec, &e->g12, &e->res, &e->dgroups, &e->rp_h, &e->rp_a};
   for (DevBuf* b : bufs) release(*b);
-  if (e->ev0) (void)hipEventDestroy(e->ev0);
-  if (e->ev1) (void)hipEventDestroy(e->ev1);
+  if (e->stream2) (void)hipStreamSynchronize(e->stream2);
+  hipEvent_t evs[] = {e->ev0, e->ev1, e->ev_fork, e->ev_join};
+  for (hipEvent_t v : evs)
+    if (v) (void)hipEventDestroy(v);
   if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream2) (void)hipStreamDestroy(e->stream2);
   delete e;
   return GNLSE_OK;
 }
@@ -1067,7 +1131,10 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
   e->ngroups = ngroups;
   e->params = *p;
   if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+      hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
     gnlse_ensemble_destroy(e);
     return fail(GNLSE_ERR_HIP, "stream/event creation failed");
   }
@@ -1105,7 +1172,7 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
   alloc(e->ws, fields * NSLOT * sizeof(double2));
   alloc(e->g12, g12s * sizeof(double));
   alloc(e->res, ress * sizeof(gnlse_instance_result));
-  alloc(e->dgroups, kEnsMaxGroups * sizeof(EnsembleGroup));
+  alloc(e->dgroups, 2 * kEnsMaxGroups * sizeof(EnsembleGroup));
   if (rc != GNLSE_OK) {
     gnlse_ensemble_destroy(e);
     return rc;
@@ -1136,13 +1203,11 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
     first += batch[i];
   }
   e->blocks = first;
-  if (hipMemcpyAsync(e->dgroups.p, e->groups.data(), ngroups * sizeof(EnsembleGroup), hipMemcpyHostToDevice,
-                     e->stream) != hipSuccess ||
-      hipStreamSynchronize(e->stream) != hipSuccess) {
+  for (int s = 0; s < ngroups; ++s) e->tabs[s] = ensemble_table_1024();
+  if ((rc = ens_replan(e)) != GNLSE_OK) {
     gnlse_ensemble_destroy(e);
-    return fail(GNLSE_ERR_HIP, "ensemble table upload failed");
+    return rc;
   }
-  e->tab = ensemble_table_1024();
   *out = e;
   return GNLSE_OK;
 }
@@ -1150,66 +1215,90 @@ int gnlse_ensemble_create(int method, const gnlse_adaptive_params* p, int ngroup
 int gnlse_ensemble_tune(gnlse_ensemble* e, int threads, int trial_attempts, int* chosen) {
   if (!e) return fail(GNLSE_ERR_ARG, "null ensemble");
   HIPCHK(hipSetDevice(e->device));
+  const EnsembleTable* pick[kEnsMaxGroups];
   if (threads != 0) {
     const EnsembleTable* t = ensemble_for_threads(threads);
     if (!t) return fail(GNLSE_ERR_ARG, "threads per realisation must be 0 (tune), 512 or 1024");
-    if (t != e->tab) drop_graph(e);
-    e->tab = t;
-    if (chosen) *chosen = t->t;
-    return GNLSE_OK;
-  }
-  if (trial_attempts < 1 || trial_attempts > 64) return fail(GNLSE_ERR_ARG, "trial_attempts must be 1..64");
-  // Trial: every realisation replays trial_attempts accepted steps of h0,
-  // a fixed amount of work per candidate; outputs are overwritten by the
-  // next run.
-  std::vector<double> rh(trial_attempts, e->params.h0);
-  std::vector<unsigned char> ra(trial_attempts, 1);
-  RET_IF(upload_raw(e->device, e->stream, e->rp_h, rh.data(), rh.size() * sizeof(double)));
-  RET_IF(upload_raw(e->device, e->stream, e->rp_a, ra.data(), ra.size()));
-  std::vector<EnsembleGroup> trial = e->groups;
-  for (EnsembleGroup& g : trial) {
-    g.prm.replay_h = (const double*)e->rp_h.p;
-    g.prm.replay_accept = (const unsigned char*)e->rp_a.p;
-    g.prm.replay_len = trial_attempts;
-  }
-  DevBuf tg;
-  RET_IF(ensure(tg, kEnsMaxGroups * sizeof(EnsembleGroup)));
-  int rc = GNLSE_OK;
-  if (hipMemcpyAsync(tg.p, trial.data(), e->ngroups * sizeof(EnsembleGroup), hipMemcpyHostToDevice, e->stream) !=
-      hipSuccess)
-    rc = fail(GNLSE_ERR_HIP, "trial table upload failed");
-  const EnsembleTable* cands[] = {ensemble_table_512(), ensemble_table_1024()};
-  const EnsembleTable* best = nullptr;
-  double best_ms = 0.0;
-  const EnsembleTable* keep = e->tab;
-  for (const EnsembleTable* t : cands) {
-    if (rc != GNLSE_OK) break;
-    e->tab = t;
-    double ms = 0.0;
-    for (int rep = 0; rep < 2 && rc == GNLSE_OK; ++rep) {  // the first run loads the code object
-      rc = ensemble_time(e, (const EnsembleGroup*)tg.p, false);
-      ms = e->last_ms;
+    for (int s = 0; s < e->ngroups; ++s) pick[s] = t;
+  } else {
+    if (trial_attempts < 1 || trial_attempts > 64) return fail(GNLSE_ERR_ARG, "trial_attempts must be 1..64");
+    // Trial: each group alone, at each compiled thread count, replays
+    // trial_attempts accepted steps of h0 in every realisation (a fixed
+    // amount of work per candidate); outputs are overwritten by the next run.
+    std::vector<double> rh(trial_attempts, e->params.h0);
+    std::vector<unsigned char> ra(trial_attempts, 1);
+    RET_IF(upload_raw(e->device, e->stream, e->rp_h, rh.data(), rh.size() * sizeof(double)));
+    RET_IF(upload_raw(e->device, e->stream, e->rp_a, ra.data(), ra.size()));
+    std::vector<EnsembleGroup> trial = e->groups;
+    for (EnsembleGroup& g : trial) {
+      g.prm.replay_h = (const double*)e->rp_h.p;
+      g.prm.replay_accept = (const unsigned char*)e->rp_a.p;
+      g.prm.replay_len = trial_attempts;
     }
-    if (rc == GNLSE_OK && (!best || ms < best_ms)) {
-      best = t;
-      best_ms = ms;
+    const EnsembleTable* cands[] = {ensemble_table_512(), ensemble_table_1024()};
+    DevBuf tg;
+    int rc = GNLSE_OK;
+    double best_ms[kEnsMaxGroups] = {};
+    double total = 0.0;
+    for (int s = 0; s < e->ngroups && rc == GNLSE_OK; ++s) {
+      pick[s] = nullptr;
+      bool sel[kEnsMaxGroups] = {};
+      sel[s] = true;
+      for (const EnsembleTable* t : cands) {
+        const EnsembleTable* tabs[kEnsMaxGroups];
+        for (int q = 0; q < kEnsMaxGroups; ++q) tabs[q] = t;
+        EnsLaunch L[2];
+        int nl = 0;
+        rc = ens_plan(e, trial, tabs, sel, tg, L, nl);
+        double ms = 0.0;
+        for (int rep = 0; rep < 2 && rc == GNLSE_OK; ++rep) {  // the first run loads the code object
+          rc = ensemble_time(e, L, nl, (const EnsembleGroup*)tg.p, false);
+          ms = e->last_ms;
+        }
+        if (rc != GNLSE_OK) break;
+        if (!pick[s] || ms < best_ms[s]) {
+          pick[s] = t;
+          best_ms[s] = ms;
+        }
+      }
+      total += best_ms[s];
     }
+    release(tg);
+    if (rc != GNLSE_OK) return rc;
+    e->last_ms = total;
   }
-  release(tg);
-  if (rc != GNLSE_OK) {
-    e->tab = keep;
-    return rc;
+  bool same = true;
+  for (int s = 0; s < e->ngroups; ++s) same = same && pick[s] == e->tabs[s];
+  if (!same) {
+    drop_graph(e);
+    for (int s = 0; s < e->ngroups; ++s) e->tabs[s] = pick[s];
+    RET_IF(ens_replan(e));
   }
-  if (best != keep) drop_graph(e);
-  e->tab = best;
-  e->last_ms = best_ms;
-  if (chosen) *chosen = best->t;
+  if (chosen) *chosen = e->tabs[0]->t;
   return GNLSE_OK;
 }
 
 int gnlse_ensemble_threads(gnlse_ensemble* e, int* threads) {
   if (!e || !threads) return fail(GNLSE_ERR_ARG, "null argument");
-  *threads = e->tab->t;
+  *threads = e->tabs[0]->t;
+  return GNLSE_OK;
+}
+
+int gnlse_ensemble_set_group_threads(gnlse_ensemble* e, int group, int threads) {
+  if (!e || group < 0 || group >= e->ngroups) return fail(GNLSE_ERR_ARG, "bad ensemble or group");
+  const EnsembleTable* t = ensemble_for_threads(threads);
+  if (!t) return fail(GNLSE_ERR_ARG, "threads per realisation must be 512 or 1024");
+  HIPCHK(hipSetDevice(e->device));
+  const int s = e->slot[group];
+  if (e->tabs[s] == t) return GNLSE_OK;
+  drop_graph(e);
+  e->tabs[s] = t;
+  return ens_replan(e);
+}
+
+int gnlse_ensemble_group_threads(gnlse_ensemble* e, int group, int* threads) {
+  if (!e || !threads || group < 0 || group >= e->ngroups) return fail(GNLSE_ERR_ARG, "bad ensemble, group or output");
+  *threads = e->tabs[e->slot[group]]->t;
   return GNLSE_OK;
 }
 
@@ -1237,7 +1326,7 @@ int gnlse_ensemble_run(gnlse_ensemble* e, int use_graph) {
   HIPCHK(hipSetDevice(e->device));
   if (use_graph && !e->exec) {
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t r = ensemble_enqueue(e, (const EnsembleGroup*)e->dgroups.p);
+    hipError_t r = ensemble_enqueue(e, e->launch, e->nlaunch, (const EnsembleGroup*)e->dgroups.p);
     hipGraph_t g = nullptr;
     hipError_t r2 = hipStreamEndCapture(e->stream, &g);
     if (r != hipSuccess || r2 != hipSuccess) {
@@ -1250,7 +1339,7 @@ int gnlse_ensemble_run(gnlse_ensemble* e, int use_graph) {
       return fail(GNLSE_ERR_HIP, "ensemble graph instantiation failed");
     }
   }
-  return ensemble_time(e, (const EnsembleGroup*)e->dgroups.p, use_graph != 0);
+  return ensemble_time(e, e->launch, e->nlaunch, (const EnsembleGroup*)e->dgroups.p, use_graph != 0);
 }
 
 int gnlse_ensemble_output(gnlse_ensemble* e, int group, double* A_out, double* g12, gnlse_instance_result* results) {

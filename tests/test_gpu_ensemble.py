@@ -153,20 +153,56 @@ def test_ensemble_graph_replays_match_eager(W):
 
 
 def test_ensemble_autotune(W):
-    """The tuner keeps one of the compiled shapes, and runs with it are the
-    runs of that shape."""
+    """The tuner keeps one of the compiled shapes for each grid, and runs with
+    it are the runs of those shapes: each group's output equals, bit for bit,
+    that of a run with every group at the group's thread count."""
     groups = ((16384, 2), (8192, 2), (4096, 2))
     inputs, _ = oracle_inputs(groups)
     ens, _ = W.noise_ensemble(groups, length=L_FULL / 64, inputs=inputs)
     chosen = ens.tune(0)
-    print(f"autotune chose {chosen} threads per realisation ({ens.last_ms():.2f} ms trial)", flush=True)
-    assert chosen in (512, 1024) and ens.threads == chosen
+    per = [ens.group_threads(g) for g in range(3)]
+    print(f"autotune chose {per} threads per realisation ({ens.last_ms():.2f} ms trial)", flush=True)
+    assert chosen in (512, 1024) and ens.threads == chosen == per[0]
+    assert all(t in (512, 1024) for t in per)
     ens.run()
     a = [ens.output(g)["A"] for g in range(3)]
-    ens.tune(chosen)
+    for T in (512, 1024):
+        ens.tune(T)
+        assert [ens.group_threads(g) for g in range(3)] == [T] * 3
+        ens.run()
+        for g in range(3):
+            if per[g] == T:
+                assert np.array_equal(ens.output(g)["A"], a[g]), f"group {g} at {T}"
+
+
+def test_ensemble_two_launches_and_their_graph(W):
+    """Grids at different thread counts run in two launches side by side (the
+    second on its own stream, joined before the coherence kernels): each
+    group equals its single-shape run bit for bit, and replays of the
+    captured graph (fork and join included) equal the eager run."""
+    groups = ((16384, 2), (8192, 3), (4096, 2))
+    inputs, _ = oracle_inputs(groups)
+    ens, _ = W.noise_ensemble(groups, length=L_FULL / 64, inputs=inputs)
+    ref = {}
+    for T in (512, 1024):
+        ens.tune(T)
+        ens.run()
+        ref[T] = [ens.output(g) for g in range(3)]
+    per = (1024, 512, 512)
+    for g, T in enumerate(per):
+        ens.set_group_threads(g, T)
+    assert [ens.group_threads(g) for g in range(3)] == list(per)
     ens.run()
-    for g in range(3):
-        assert np.array_equal(ens.output(g)["A"], a[g])
+    eager = [ens.output(g) for g in range(3)]
+    for g, T in enumerate(per):
+        assert np.array_equal(eager[g]["A"], ref[T][g]["A"]), f"group {g}"
+        assert np.array_equal(eager[g]["g12"], ref[T][g]["g12"]), f"group {g}"
+    for rep in range(3):
+        ens.run(graph=True)
+        for g in range(3):
+            o = ens.output(g)
+            assert np.array_equal(o["A"], eager[g]["A"]), f"replay {rep} group {g}"
+            assert np.array_equal(o["g12"], eager[g]["g12"]), f"replay {rep} group {g}"
 
 
 def test_noise_ensemble_full_size(W):
