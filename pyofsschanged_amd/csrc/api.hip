@@ -594,8 +594,17 @@ static int launch_adaptive(gnlse_ctx* c, int method, int64_t batch, const Adapti
   RET_IF(check_pump(c, batch));
   // Two workgroups per instance (the grid's default schedule, pairing on)
   // when the operators have that kernel and every workgroup of the batch
-  // fits on the device at once; else one workgroup per instance.
-  if (c->pairing && c->sched == 0 && c->tab->adaptive_pair && !c->ops.wdm) {
+  // fits on the device at once; else one workgroup per instance. The
+  // 16384-point Kerr kernels share the device's memory bandwidth worst:
+  // pairs there gain only while they fill at most three quarters of the CUs
+  // (measured: 0.90x the time at 16 instances, 0.99x at 96, 1.06x at 128).
+  bool pair = c->pairing && c->sched == 0 && c->tab->adaptive_pair && !c->ops.wdm;
+  if (pair && c->ops.nl_kind == NL_KERR && c->n >= 16384) {
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    pair = 8 * batch <= 3 * (int64_t)cus;
+  }
+  if (pair) {
     RET_IF(ensure(c->ws, (size_t)2 * batch * kPairSlots * c->n * sizeof(double2)));
     RET_IF(ensure(c->pair_flags, (size_t)2 * batch * sizeof(unsigned)));
     HIPCHK(hipMemsetAsync(c->pair_flags.p, 0, (size_t)2 * batch * sizeof(unsigned), c->stream));
