@@ -266,6 +266,36 @@ def test_kerr_amplifier_adaptive_and_fixed(lib, N, length):
     assert err <= FIELD_TOL
 
 
+@pytest.mark.parametrize("N", [8192, 16384])
+def test_kerr_soliton_large_grids(lib, N):
+    """The gain-free Kerr kernels (written-out rk4ip, no gain branch) on the
+    large grids through the single-grid launches: adaptive rk4ip of the N = 2
+    soliton over 1/16 of its period with two workgroups per instance and with
+    one, and 64 fixed rk4ip steps, each against the oracle."""
+    dom, A, fib = soliton_c1(N)  # the whole period: 155 attempted steps
+    ref, f = adaptive_oracle(dom, A, fib, "ark4ip", 1e-6)
+    log = f.stepper.attempt_log
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"])
+    ctx = engine_context(dom, lin, nl, 0)
+    for pair in (True, False):
+        ctx.set_pairing(pair)
+        out = ctx.propagate_adaptive(0, A, adaptive_params(lib, fib, 1e-6), log_cap=len(log) + 8)
+        r = out["results"][0]
+        err = rel_l2(out["A"], ref)
+        print(f"Kerr soliton N={N} pairing {pair}: attempts {r['n_attempts']} (oracle {len(log)}), "
+              f"rel L2 {err:.3e}", flush=True)
+        assert r["status"] == 0 and r["n_attempts"] == len(log)
+        assert np.array_equal(out["log_accept"][0, :len(log)], np.array([a[2] for a in log], dtype=np.uint8))
+        assert err <= FIELD_TOL
+    steps = 64
+    fo = O.Fibre(length=fib["length"], beta=fib["beta"], gamma=fib["gamma"], method="rk4ip", total_steps=steps)
+    ref_f = fo(dom, A.copy())
+    got, _, _ = ctx.propagate_fixed(0, A, fib["length"] / steps, steps)
+    err = rel_l2(got, ref_f)
+    print(f"Kerr soliton N={N} fixed: rel L2 {err:.3e}", flush=True)
+    assert err <= FIELD_TOL
+
+
 def test_adaptive_nan_input_raises_small_step(lib):
     """stepper.py:234-237, 349-351: NaN fields are rejected until h < 1e-37."""
     dom, A, fib = soliton_c1(1024)
