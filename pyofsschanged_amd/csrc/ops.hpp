@@ -271,8 +271,9 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
 // bit 14 (kCfgRamanUnroll): the two transforms of raman_real at two call
 // sites. Both trade code size for fewer register copies where the loops'
 // paths merge: 8192 points at 512 threads spill 100 B per thread instead of
-// 264 B, and the headline attempt drops from 501 to 443 us (with 8-point
-// stage blocks).
+// 264 B, and an attempted step of the 8192-point Raman + self-steepening
+// kernel (256 instances) drops from 501 to 443 us (with 8-point stage
+// blocks).
 constexpr int kCfgNoGain = 0x400, kCfgLine = 0x1000, kCfgKerrFree = kCfgNoGain | kCfgLine;
 constexpr int kCfgSpUnroll = 0x2000, kCfgRamanUnroll = 0x4000, kCfgUnrolled = kCfgSpUnroll | kCfgRamanUnroll;
 template <int N, int T, int KIND, bool YB = false, int CFG = 0>
