@@ -65,17 +65,17 @@ def oracle_inputs(groups, base_seed=1000):
     return out, doms
 
 
-def oracle_run(dom, A, length):
-    f = O.Fibre(length=length, beta=[0.0, 0.0, -20.0], gamma=1.3, method="ark4ip", local_error=1e-6)
+def oracle_run(dom, A, length, method="ark4ip"):
+    f = O.Fibre(length=length, beta=[0.0, 0.0, -20.0], gamma=1.3, method=method, local_error=1e-6)
     out = f(dom, A.copy())
     return out, len(f.stepper.attempt_log)
 
 
-def field_bar(dom, A, length, ref):
+def field_bar(dom, A, length, ref, method="ark4ip"):
     """max(1e-9, 10 s) with s the oracle's sensitivity to a 1e-13 relative
     perturbation of its input."""
     rng = np.random.default_rng(0)
-    moved, _ = oracle_run(dom, A * (1 + 1e-13 * rng.standard_normal(A.size)), length)
+    moved, _ = oracle_run(dom, A * (1 + 1e-13 * rng.standard_normal(A.size)), length, method=method)
     s = rel_l2(moved, ref)
     return max(FIELD_TOL, 10 * s), s
 
@@ -150,6 +150,36 @@ def test_ensemble_graph_replays_match_eager(W):
             assert np.array_equal(o["A"], eager[g]["A"]), f"replay {rep} group {g}"
             assert np.array_equal(o["g12"], eager[g]["g12"]), f"replay {rep} group {g}"
         ctx.fft(inputs[0], +1)  # eager work on the device between replays
+
+
+def test_ensemble_ss_symmetric_matches_oracle(W):
+    """The adaptive symmetric split-step (ass_symmetric) in the gain-free Kerr
+    ensemble kernels, two grids at their tuned thread counts: each
+    realisation against the oracle's Fibre run of its input, identical
+    attempt counts and fields within 1e-9. The 16384-sample realisations are
+    noise-free sech pulses (two amplitudes): with one-photon-per-mode noise
+    the controller of this third-order method is ill-conditioned on that
+    grid (the oracle's own output moves by 1e-6, and its attempt count by
+    one, under a 1e-13 perturbation of its input, while the device's replay
+    of the oracle's step sequence matches it to 2e-12)."""
+    groups = ((16384, 2), (8192, 2))
+    length = L_FULL / 8
+    noisy, doms = oracle_inputs(groups)
+    A0 = O.Sech(peak_power=4 * 20.0 / 1.3, width=1.0).generate(doms[0].t)
+    inputs = [np.stack([A0, 1.05 * A0]), noisy[1]]
+    ens, _ = W.noise_ensemble(groups, length=length, inputs=inputs, method="ass_symmetric")
+    ens.tune(0)
+    ens.run()
+    for g, (N, M) in enumerate(groups):
+        out = ens.output(g)
+        for r in range(M):
+            ref, n_att = oracle_run(doms[g], inputs[g][r], length, method="ass_symmetric")
+            res = out["results"][r]
+            err = rel_l2(out["A"][r], ref)
+            print(f"  ss_symmetric N={N} r={r} ({ens.group_threads(g)} threads): attempts {res['n_attempts']} "
+                  f"(oracle {n_att}), rel L2 {err:.2e}", flush=True)
+            assert res["status"] == 0 and res["n_attempts"] == n_att
+            assert err <= FIELD_TOL
 
 
 def test_ensemble_autotune(W):
