@@ -182,6 +182,42 @@ def test_ensemble_ss_symmetric_matches_oracle(W):
             assert err <= FIELD_TOL
 
 
+def test_ensemble_with_saturable_gain_matches_oracle(W):
+    """Kerr groups with the saturable gain (the ensemble kernels that keep
+    the gain branch; gain-free groups take the gain-free ones), 8192 and
+    4096 samples, 3 dB over 1/8 of the soliton period with E_sat = 0.2 nJ
+    and a 40 nm Lorentzian at 1550 nm: each realisation against the oracle's
+    Fibre run of its input."""
+    from pyofsschanged_amd import Domain, Fibre
+    from pyofsschanged_amd._lib import Ensemble
+    groups = ((8192, 2), (4096, 2))
+    length = L_FULL / 8
+    amp = dict(small_signal_gain=3.0, E_sat=0.2, lamb0=1550.0, bandwidth=40.0)
+    inputs, doms = oracle_inputs(groups)
+    fibres = []
+    for N, M in groups:
+        fib = Fibre(length=length, beta=[0.0, 0.0, -20.0], gamma=1.3, method="ark4ip", local_error=1e-6, **amp)
+        fib.prepare(Domain(1, N, 100.0, C_NM_PS / 1550.0))
+        fibres.append(fib)
+    ens = Ensemble(fibres[0].stepper.solver.method_id, fibres[0].stepper.adaptive_params(),
+                   [(f._device_context(), M) for f, (N, M) in zip(fibres, groups)])
+    for g in range(len(groups)):
+        ens.set_input(g, inputs[g])
+    ens.tune(0)
+    ens.run()
+    for g, (N, M) in enumerate(groups):
+        out = ens.output(g)
+        for r in range(M):
+            f = O.Fibre(length=length, beta=[0.0, 0.0, -20.0], gamma=1.3, method="ark4ip", local_error=1e-6, **amp)
+            ref = f(doms[g], inputs[g][r].copy())
+            n_att = len(f.stepper.attempt_log)
+            res = out["results"][r]
+            err = rel_l2(out["A"][r], ref)
+            print(f"  gain N={N} r={r}: attempts {res['n_attempts']} (oracle {n_att}), rel L2 {err:.2e}", flush=True)
+            assert res["status"] == 0 and res["n_attempts"] == n_att
+            assert err <= FIELD_TOL
+
+
 def test_ensemble_autotune(W):
     """The tuner keeps one of the compiled shapes for each grid, and runs with
     it are the runs of those shapes: each group's output equals, bit for bit,
