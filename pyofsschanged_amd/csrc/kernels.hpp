@@ -572,6 +572,8 @@ hipError_t launch_pair(hipStream_t s, const Ops& o, const AdaptiveParams& p, lon
       hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, (CC) | kCfgKerrFree>), __VA_ARGS__);                   \
     else if ((ops).nl_kind == NL_KERR) hipLaunchKernelGGL((KERNEL, NL_KERR, false, 0, CC>), __VA_ARGS__); \
     else if ((ops).nl_kind == NL_ALL) hipLaunchKernelGGL((KERNEL, NL_ALL, false, 0, CC>), __VA_ARGS__); \
+    else if ((ops).gain == GAIN_NONE)                                                                   \
+      hipLaunchKernelGGL((KERNEL, NL_GENERIC, false, 0, (CC) | kCfgNoGain>), __VA_ARGS__);                  \
     else hipLaunchKernelGGL((KERNEL, NL_GENERIC, false, 0, CC>), __VA_ARGS__);                          \
   } while (0)
 

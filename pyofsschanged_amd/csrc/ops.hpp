@@ -266,7 +266,9 @@ __device__ __forceinline__ void wg_sum_n(double (&v)[M], double* buf) {
 // (kCfgLine): rk4ip runs written out stage by stage (rk4ip_line). The Kerr
 // launches without gain set both (kCfgKerrFree): the runtime gain branch and
 // the merging paths of the 7-stage loop cost the 16384-point Kerr kernels
-// half their speed in register spills. Bit 13 (kCfgSpUnroll): the twelve
+// half their speed in register spills. The gain-free launches of the other
+// time-domain operators (self-steepening, T_R Raman) set kCfgNoGain: 16384
+// points, 35-39 % less time per attempted step. Bit 13 (kCfgSpUnroll): the twelve
 // passes of rk4ip_sp unrolled (each pass compiled for its stage and phase);
 // bit 14 (kCfgRamanUnroll): the two transforms of raman_real at two call
 // sites. Both trade code size for fewer register copies where the loops'

@@ -296,6 +296,40 @@ def test_kerr_soliton_large_grids(lib, N):
     assert err <= FIELD_TOL
 
 
+GENERIC_KINDS = [dict(self_steepening=True), dict(raman_scattering=True),
+                 dict(self_steepening=True, raman_scattering=True)]
+
+
+@pytest.mark.parametrize("flags", GENERIC_KINDS, ids=lambda d: "+".join(sorted(d)))
+@pytest.mark.parametrize("N", [4096, 16384])
+def test_generic_kinds_adaptive_and_fixed(lib, flags, N):
+    """The self-steepening and T_R Raman operators (nonlinearity.py:223-256)
+    through the gain-free propagating kernels: adaptive rk4ip over the first
+    1e-5 km (4096 samples) or 4e-6 km (16384) of the C2 fibre (the oracle's
+    attempt sequence) and 50 fixed rk4ip steps, against the oracle."""
+    dom, A, fib = supercontinuum_c2(N)
+    length = 1e-5 if N == 4096 else 4e-6
+    f = O.Fibre(length=length, beta=fib["beta"], gamma=fib["gamma"], method="ark4ip", local_error=1e-6, **flags)
+    ref = f(dom, A.copy())
+    log = f.stepper.attempt_log
+    lin, nl = oracle_ops(dom, beta=fib["beta"], gamma=fib["gamma"], **flags)
+    ctx = engine_context(dom, lin, nl, nl_kind(**flags))
+    out = ctx.propagate_adaptive(0, A, adaptive_params(lib, dict(fib, length=length), 1e-6), log_cap=len(log) + 8)
+    r = out["results"][0]
+    err = rel_l2(out["A"], ref)
+    print(f"{flags} N={N}: attempts {r['n_attempts']} (oracle {len(log)}), rel L2 {err:.3e}", flush=True)
+    assert r["status"] == 0 and r["n_attempts"] == len(log)
+    assert np.array_equal(out["log_accept"][0, :len(log)], np.array([a[2] for a in log], dtype=np.uint8))
+    assert err <= FIELD_TOL
+    steps = 50
+    fo = O.Fibre(length=length, beta=fib["beta"], gamma=fib["gamma"], method="rk4ip", total_steps=steps, **flags)
+    ref_f = fo(dom, A.copy())
+    got, _, _ = ctx.propagate_fixed(0, A, length / steps, steps)
+    err = rel_l2(got, ref_f)
+    print(f"{flags} N={N} fixed: rel L2 {err:.3e}", flush=True)
+    assert err <= FIELD_TOL
+
+
 def test_adaptive_nan_input_raises_small_step(lib):
     """stepper.py:234-237, 349-351: NaN fields are rejected until h < 1e-37."""
     dom, A, fib = soliton_c1(1024)
