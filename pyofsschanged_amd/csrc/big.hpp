@@ -493,7 +493,7 @@ __device__ __forceinline__ void big_rk4ip(BigDev<N, YB>& d, const double2* in, d
     const double2 ai = sAI[p];
     const double2 k2 = sU[p];
     const double2 s = cadd(sK0[p], cscale(cadd(sK1[p], k2), 2.0));
-    sK0[p] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+    sK0[p] = cadd(ai, make_double2(s.x * kSixth, s.y * kSixth));  // A_I + (k0 + 2(k1+k2))/6
     sU[p] = cadd(ai, k2);                                   // A_I + k2
   }
   d.linear(sU, sU, E, hh);
@@ -503,7 +503,7 @@ __device__ __forceinline__ void big_rk4ip(BigDev<N, YB>& d, const double2* in, d
   for (int q = 0; q < G::PT; ++q) {
     const unsigned p = d.pos(q);
     const double2 k3 = sK1[p];
-    out[p] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), out[p]);
+    out[p] = cadd(make_double2(k3.x * kSixth, k3.y * kSixth), out[p]);
   }
 }
 
@@ -555,7 +555,7 @@ __device__ __forceinline__ void big_ss_sym_rk4(BigDev<N, YB>& d, const double2* 
   for (int q = 0; q < G::PT; ++q) {
     const unsigned p = d.pos(q);
     const double2 s = cadd(cadd(sK0[p], cscale(sK1[p], 2.0)), sU[p]);
-    sU[p] = cadd(sAL[p], make_double2(s.x / 6.0, s.y / 6.0));
+    sU[p] = cadd(sAL[p], make_double2(s.x * kSixth, s.y * kSixth));
   }
   d.linear(sU, out, E, 0.5 * h);
 }

@@ -87,6 +87,10 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
 }
 
 __device__ __forceinline__ double abs2(double2 a) { return a.x * a.x + a.y * a.y; }
+// The 1/6 of the RK4 combinations as a product with the rounded constant
+// (within one unit in the last place of the quotient; an FP64 division is
+// an instruction sequence ten times longer).
+constexpr double kSixth = 1.0 / 6.0;
 
 // The dispersion factor of a linear call: E as stored, or its square (the
 // coarse step's exp(factor h/2) from the stored exp(factor h/4): the same
@@ -199,10 +203,10 @@ __device__ __forceinline__ void pw_sp_stage(double2 (&x)[P], bool fresh, double2
         w = cadd(a[j], cscale(v, 0.5));
       } else if (ST == 3) {  // k2~: S <- A_I~ + S/6; next input E (A_I~ + k2~)
         const double2 q = cadd(b[j], cscale(v, 2.0));
-        S[T * (c + j)] = cadd(a[j], make_double2(q.x / 6.0, q.y / 6.0));
+        S[T * (c + j)] = cadd(a[j], make_double2(q.x * kSixth, q.y * kSixth));
         w = cmul(e[j], cadd(a[j], v));
       } else {  // k3~: A'~ = k3~/6 + E S
-        w = cadd(make_double2(v.x / 6.0, v.y / 6.0), cmul(e[j], b[j]));
+        w = cadd(make_double2(v.x * kSixth, v.y * kSixth), cmul(e[j], b[j]));
         if (out) out[T * (c + j)] = w;
       }
       x[c + j] = conjg(w);
@@ -644,7 +648,7 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N /
         const int i = d.idx(k);
         const double2 ai = sAI[i];
         const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], x[k]), 2.0));
-        sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+        sK0[i] = cadd(ai, make_double2(s.x * kSixth, s.y * kSixth));  // A_I + (k0 + 2(k1+k2))/6
         x[k] = cadd(ai, x[k]);                                  // A_I + k2
       }
     } else if (st == 5) d.store(sK1, x);  // k3
@@ -652,7 +656,7 @@ __device__ __forceinline__ void rk4ip(Dev<N, T, K, YB, CFG>& d, double2 (&x)[N /
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         const double2 k3 = sK1[d.idx(k)];
-        x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
+        x[k] = cadd(make_double2(k3.x * kSixth, k3.y * kSixth), x[k]);
       }
     }
   }
@@ -704,7 +708,7 @@ __device__ __forceinline__ void rk4ip_line(Dev<N, T, K, YB, CFG>& d, double2 (&x
     const double2 k2 = cscale(x[k], h);
     const double2 ai = sAI[i];
     const double2 s = cadd(sK0[i], cscale(cadd(sK1[i], k2), 2.0));
-    sK0[i] = cadd(ai, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+    sK0[i] = cadd(ai, make_double2(s.x * kSixth, s.y * kSixth));  // A_I + (k0 + 2(k1+k2))/6
     x[k] = cadd(ai, k2);                                    // A_I + k2
   }
   d.linear(x, E, hh);  // k3 = h N(L(A_I + k2))
@@ -717,7 +721,7 @@ __device__ __forceinline__ void rk4ip_line(Dev<N, T, K, YB, CFG>& d, double2 (&x
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const double2 k3 = sK1[d.idx(k)];
-    x[k] = cadd(make_double2(k3.x / 6.0, k3.y / 6.0), x[k]);
+    x[k] = cadd(make_double2(k3.x * kSixth, k3.y * kSixth), x[k]);
   }
 }
 
@@ -867,7 +871,7 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB, CFG>& d, doubl
       for (int j = 0; j < C; ++j) {
         const double2 v = cscale(cmul(m[j], cscale(x[c + j], INVN)), h);
         const double2 q = cadd(b[j], cscale(v, 2.0));
-        const double2 sn = cadd(ai[j], make_double2(q.x / 6.0, q.y / 6.0));
+        const double2 sn = cadd(ai[j], make_double2(q.x * kSixth, q.y * kSixth));
         S[T * (c + j)] = sn;
         x[c + j] = cadd(ai[j], v);
         gain_acc<N, GK>(op, a, x[c + j], t + T * (c + j));
@@ -905,7 +909,7 @@ __device__ __forceinline__ void pw_sp_stage_g(Dev<N, T, KIND, YB, CFG>& d, doubl
       for (int j = 0; j < C; ++j) {
         const double2 v = cscale(cmul(m[j], cscale(x[c + j], INVN)), h);
         const double2 w =
-            cadd(make_double2(v.x / 6.0, v.y / 6.0), cscale(cmul(e[j], b[j]), gain_factor<GK>(q[j], gp4, hl)));
+            cadd(make_double2(v.x * kSixth, v.y * kSixth), cscale(cmul(e[j], b[j]), gain_factor<GK>(q[j], gp4, hl)));
         if (out) out[T * (c + j)] = w;
         x[c + j] = conjg(w);
       }
@@ -1167,7 +1171,7 @@ __device__ __forceinline__ void ss_sym_rk4(Dev<N, T, K, YB, CFG>& d, double2 (&x
       for (int k = 0; k < P; ++k) {
         const int i = d.idx(k);
         const double2 s = cadd(cadd(sK0[i], cscale(sK1[i], 2.0)), x[k]);
-        x[k] = cadd(sAL[i], make_double2(s.x / 6.0, s.y / 6.0));
+        x[k] = cadd(sAL[i], make_double2(s.x * kSixth, s.y * kSixth));
       }
     }
   }

@@ -110,7 +110,7 @@ __device__ __forceinline__ void rk4ip_wdm(Dev<N, T, K, YB>& d, const WdmSlots<N>
       const double2 k2 = cscale(x[k], h);
       const double2 a = ai[i];
       const double2 s = cadd(k0[i], cscale(cadd(k1[i], k2), 2.0));
-      k0[i] = cadd(a, make_double2(s.x / 6.0, s.y / 6.0));  // A_I + (k0 + 2(k1+k2))/6
+      k0[i] = cadd(a, make_double2(s.x * kSixth, s.y * kSixth));  // A_I + (k0 + 2(k1+k2))/6
       x[k] = cadd(a, k2);                                    // A_I + k2
     }
     d.linear(x, c ? E1 : E0, hh);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void rk4ip_wdm(Dev<N, T, K, YB>& d, const WdmSlots<N>
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       const double2 v = k3[d.idx(k)];
-      x[k] = cadd(make_double2(v.x / 6.0, v.y / 6.0), x[k]);
+      x[k] = cadd(make_double2(v.x * kSixth, v.y * kSixth), x[k]);
     }
     d.store(W.at(c, out), x);
   }
